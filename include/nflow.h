@@ -1,0 +1,214 @@
+/*
+ * nflow.h -- C ABI of the MI355X-native D3Q19 lattice-Boltzmann solver.
+ *
+ * Drop-in for the reference's time-step loop (eugenekim-mzc/NFLOW-RED-Poplar):
+ * it keeps the reference's parameter block, its decomposition API and its
+ * exported population layout, and runs the step on hand-written HIP kernels.
+ * Plain C types only; no exceptions cross this boundary.  Every function
+ * returns 0 on success and a negative NFLOW_E* code on failure (the reference
+ * calls exit(0) instead, inverseMRT.cpp:89, kernel.cu:711/979);
+ * nflow_last_error() gives the message for the calling thread.
+ *
+ * Ownership: the library owns all device memory; callers own host buffers.
+ * Reads and writes of host buffers are synchronous.  A solver, batch or
+ * server handle is used by one host thread at a time (the server's submit and
+ * poll calls are the exception: they may come from any thread).
+ *
+ * Layouts (axis order [y][z][x], y slowest, as in main.cpp:1 and
+ * lbm_codelets.cpp:52):
+ *   arF   : float [ny+2][nz+2][nx+2][19], one ghost layer per side, ghosts = 0
+ *           (the layout README:9-11 documents; the reference's own tensor is
+ *           the unpadded [ny][nz][nx][19] of main.cpp:216)
+ *   macro : float [ny][nz][nx] for rho, ux, uy, uz
+ */
+#ifndef NFLOW_H
+#define NFLOW_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NFLOW_DIMENSION 3       /* common.h:7  */
+#define NFLOW_DIRECTION_SIZE 19 /* common.h:8  */
+#define NFLOW_XDIM 0            /* common.h:13-15: axis ids of Direction[a][q] */
+#define NFLOW_ZDIM 1
+#define NFLOW_YDIM 2
+
+#define NFLOW_CAVITY 1  /* common.h:26 */
+#define NFLOW_CHANNEL 0 /* common.h:27 (POISEUILLE); here a square duct */
+
+#define NFLOW_NODE_FLUID (1 << 1) /* common.h:32-37 */
+#define NFLOW_NODE_FIXEDWALL (1 << 2)
+#define NFLOW_NODE_MOVINGWALL (1 << 3)
+#define NFLOW_NODE_INLET (1 << 4)
+#define NFLOW_NODE_OUTLET (1 << 5)
+#define NFLOW_NODE_STRUCT (1 << 6)
+
+#define NFLOW_STORAGE_F32 0  /* FP32 populations */
+#define NFLOW_STORAGE_F16 1  /* FP16 of f - w_q, FP32 arithmetic */
+#define NFLOW_STORAGE_BF16 2 /* BF16 of f - w_q, FP32 arithmetic */
+
+#define NFLOW_OK 0
+#define NFLOW_EINVAL (-1)  /* bad argument */
+#define NFLOW_ENOMEM (-2)  /* device or host allocation failed */
+#define NFLOW_EDEVICE (-3) /* HIP runtime error */
+#define NFLOW_ESTATE (-4)  /* call out of order (e.g. run before initialize) */
+#define NFLOW_EBUDGET (-5) /* request larger than the server's HBM budget */
+#define NFLOW_ENOTFOUND (-6)
+
+/* Parameter block.  Superset of Inputval (struct.h:78-100): the fields the
+ * step uses, plus storage precision and the channel inlet description. */
+typedef struct nflow_input {
+  int h;                 /* Inputval.h: reference length (main.cpp:59) */
+  int nx, ny, nz;        /* interior nodes per axis */
+  int maxitr;            /* steps the driver runs after initialisation */
+  int nwrite;            /* output period of the driver (kernel.cu:205) */
+  int casetype;          /* NFLOW_CAVITY or NFLOW_CHANNEL */
+  int enable_mrt;        /* 1: MRT collision, 0: SRT/BGK */
+  int enable_les;        /* Smagorinsky eddy viscosity */
+  int storage;           /* NFLOW_STORAGE_* */
+  int init_lid_velocity; /* 1: lid nodes start at the lid velocity (lbm_codelets.cpp:215-220) */
+  double tau;            /* m_dTau */
+  double smagorinsky;    /* m_dSmagorinsky (Cs) */
+  double ref_den;        /* m_dRefDen */
+  double ref_vel[3];     /* m_dRefVelx/y/z: lid velocity */
+  double w_eps, w_epsj, w_xx;   /* dWeps, dWepsj, dWXX */
+  double inlet_peak;     /* channel: peak inlet velocity of the parabola */
+  double perturb;        /* channel: relative amplitude of the seeded inlet perturbation */
+  unsigned long long seed;
+} nflow_input;
+
+typedef struct nflow_solver nflow_solver;
+typedef struct nflow_batch nflow_batch;
+typedef struct nflow_server nflow_server;
+
+/* Reference defaults of main.cpp:59-85 (10^3 cavity, 199 steps, MRT+LES, Cs = 0). */
+void nflow_input_defaults(nflow_input* in);
+
+/* ---- decomposition (replaces decomposition.cpp) --------------------------
+ * decomposition.cpp:98-161 with the XDECOMP/YDECOMP/ZDECOMP macros of
+ * common.h:17-19 as arguments.  Fills slice[t][axis][start,end) (axis 0 = x,
+ * 1 = y, 2 = z) and returns the number of tiles, or a negative code. */
+int nflow_decomposition(const nflow_input* in, int xdecomp, int ydecomp, int zdecomp, int slice[][3][2],
+                        int max_tiles);
+/* slice_chunk, decomposition.cpp:66-94: returns 1 when the tile is empty. */
+int nflow_slice_chunk(unsigned tile, const int slice[][3][2], int* xs, int* xe, int* ys, int* ye, int* zs, int* ze,
+                      int* chunk_nx, int* chunk_ny, int* chunk_nz, int* chunk_len);
+/* set_move_decomp, decomposition.cpp:17-62: per-tile neighbour offsets into
+ * the tile's ghost-padded sub-block. */
+int nflow_set_move_decomp(int ntile, const int slice[][3][2], int move[][NFLOW_DIRECTION_SIZE]);
+/* The moment basis M of main.cpp:149-171. */
+void nflow_mrt_matrix(float m[NFLOW_DIRECTION_SIZE][NFLOW_DIRECTION_SIZE]);
+/* inverseMRT, inverseMRT.cpp:8-92: Gauss-Jordan with partial pivoting in
+ * double; returns NFLOW_EINVAL for a singular matrix instead of exit(0). */
+int nflow_inverse_mrt(const float m[NFLOW_DIRECTION_SIZE][NFLOW_DIRECTION_SIZE],
+                      float minv[NFLOW_DIRECTION_SIZE][NFLOW_DIRECTION_SIZE]);
+
+/* ---- one lattice (replaces initialize.cpp + runSimulation.cpp + the
+ *      Engine run of main.cpp:278-317) ------------------------------------
+ * nflow_create: the tiles must partition the box exactly; they become the
+ * work partition of the kernels (results do not depend on them). */
+int nflow_create(const nflow_input* in, int ntile, const int slice[][3][2], nflow_solver** out);
+/* Initialize compute set (initialize.cpp:23-98, lbm_codelets.cpp:111-242). */
+int nflow_initialize(nflow_solver* s);
+/* Repeat(nsteps, runSimulation step) (main.cpp:291, runSimulation.cpp:164-165). */
+int nflow_run(nflow_solver* s, int nsteps);
+int nflow_synchronize(nflow_solver* s);
+long long nflow_steps_done(const nflow_solver* s);
+/* readTensor("D2H-arF") (main.cpp:317): post-stream, post-boundary populations. */
+int nflow_read_arF(nflow_solver* s, float* dst);
+/* writeTensor("H2D-arF") (main.cpp:303); also the checkpoint/restart path.
+ * rho/u restart from the moments of src; the step counter restarts at 0. */
+int nflow_write_arF(nflow_solver* s, const float* src);
+/* CurrDen / CurrVelx/y/z after the last step (Macroscopic, lbm_codelets.cpp:248-318). */
+int nflow_read_macro(nflow_solver* s, float* rho, float* ux, float* uy, float* uz);
+/* Node flags and direction masks (keSetNodeType / keInitialNodeDirection), [ny][nz][nx]. */
+int nflow_read_node_type(nflow_solver* s, int* node_type, int* direction_state);
+/* FP64 reductions on the device: total mass, momentum and the residual
+ * sum |PrevDen - CurrDen| of kernel.cu:241-245. */
+int nflow_diagnostics(nflow_solver* s, double* mass, double mom[3], double* drho_l1);
+/* Device time of the last nflow_run, per step (HIP events on the solver's stream). */
+int nflow_timing(nflow_solver* s, double* ms_per_step, int* steps_timed);
+void nflow_destroy(nflow_solver* s);
+const char* nflow_last_error(void);
+
+/* ---- many lattices (no reference counterpart) ------------------------------
+ * Independent lattices of any sizes advanced together: one launch per time
+ * step covers every lattice; steps are replayed from a captured HIP graph;
+ * the launch shape comes from nflow_batch_autotune. All lattices share one
+ * storage precision (the first input's). */
+typedef struct nflow_tuning {
+  int block;       /* threads per workgroup: 64, 128, 256 or 512 */
+  int xcd_order;   /* 1: each XCD receives a contiguous range of work items */
+  int persistent;  /* 0: one workgroup per work item; k > 0: k workgroups per CU, grid-stride */
+  int graph_steps; /* steps per captured graph (even), 0 = no graph */
+  double ms_per_step; /* filled by the autotuner */
+} nflow_tuning;
+
+int nflow_batch_create(int count, const nflow_input* inputs, nflow_batch** out);
+int nflow_batch_initialize(nflow_batch* b);
+int nflow_batch_autotune(nflow_batch* b, int steps_per_trial, nflow_tuning* chosen);
+int nflow_batch_set_tuning(nflow_batch* b, const nflow_tuning* t);
+int nflow_batch_get_tuning(nflow_batch* b, nflow_tuning* t);
+int nflow_batch_run(nflow_batch* b, int nsteps);
+int nflow_batch_synchronize(nflow_batch* b);
+int nflow_batch_read_arF(nflow_batch* b, int idx, float* dst);
+int nflow_batch_read_macro(nflow_batch* b, int idx, float* rho, float* ux, float* uy, float* uz);
+int nflow_batch_diagnostics(nflow_batch* b, int idx, double* mass, double mom[3], double* drho_l1);
+int nflow_batch_timing(nflow_batch* b, double* ms_per_step, int* steps_timed);
+long long nflow_batch_nodes(nflow_batch* b);
+size_t nflow_batch_device_bytes(nflow_batch* b);
+void nflow_batch_destroy(nflow_batch* b);
+
+/* ---- serving (no reference counterpart) ------------------------------------
+ * A continuous-batching loop on a host thread: requests are admitted in
+ * arrival order as their device footprint fits the HBM budget and a slot is
+ * free (smaller later requests may backfill), stay resident until their step
+ * count is reached, and share every step launch with the other resident
+ * lattices.  Results are returned in completion order. */
+typedef struct nflow_request {
+  nflow_input input;        /* storage is overridden by the server's */
+  int nsteps;
+  int want_fields;          /* 1: keep rho/u on the host for nflow_server_fetch_macro */
+  unsigned long long tag;   /* caller's id, echoed in the result */
+} nflow_request;
+
+typedef struct nflow_result {
+  unsigned long long tag;
+  int status;               /* 0 ok, negative NFLOW_E* */
+  int nx, ny, nz;
+  long long steps;
+  double submit_s, admit_s, done_s; /* seconds since the server started */
+  double mass, mom[3], drho_l1;     /* FP64 diagnostics of the final state */
+} nflow_result;
+
+typedef struct nflow_server_stats {
+  long long submitted, admitted, completed, rejected;
+  long long node_updates;   /* sum over completed requests of nodes * steps */
+  long long launches;
+  double busy_s;            /* wall time with at least one resident lattice */
+  size_t peak_bytes;        /* peak device bytes held by resident lattices */
+  int peak_resident;
+} nflow_server_stats;
+
+int nflow_server_create(size_t hbm_budget_bytes, int storage, int max_resident, int steps_per_tick,
+                        nflow_server** out);
+int nflow_server_submit(nflow_server* s, const nflow_request* r);
+int nflow_server_poll(nflow_server* s, nflow_result* out, int max_results, int wait_ms);
+int nflow_server_fetch_macro(nflow_server* s, unsigned long long tag, float* rho, float* ux, float* uy, float* uz);
+int nflow_server_drain(nflow_server* s);
+int nflow_server_stats_get(nflow_server* s, nflow_server_stats* st);
+void nflow_server_destroy(nflow_server* s);
+
+/* ---- measurement helpers ---------------------------------------------------
+ * Device copy of `bytes` bytes split over `streams` equal arrays (a streaming
+ * ceiling for the population traffic), timed with HIP events. */
+int nflow_copy_bandwidth(size_t bytes, int streams, int reps, double* gbps, double* ms_per_copy);
+int nflow_device_info(char* name, int name_len, int* cu_count, size_t* hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

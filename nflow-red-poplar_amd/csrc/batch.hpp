@@ -1,0 +1,101 @@
+// batch.hpp -- host runtime: lattices resident in device memory, the work
+// list that maps workgroups onto them, graph-captured stepping, export.
+#pragma once
+#include <cstddef>
+#include <vector>
+
+#include "lattice.hpp"
+#include "nflow.h"
+
+namespace nflow {
+
+// A sub-block [x0,x1) x [y0,y1) x [z0,z1) of one lattice (an IPU tile of the
+// reference's decomposition); workgroups never straddle two tiles.
+struct Tile {
+  int x0, x1, y0, y1, z0, z1;
+};
+
+struct HostLattice {
+  nflow_input in{};
+  std::vector<Tile> tiles;
+  size_t bytes = 0;      // device bytes of this lattice
+  void* mem = nullptr;   // base of its device memory
+  bool own_mem = false;  // allocated with hipMalloc by this lattice
+  LatticeDev dev{};
+  long long steps_done = 0;
+  int cur = 0;  // buffer holding the newest post-collision state
+  bool initialized = false;
+};
+
+int validate_input(const nflow_input& in, int storage);
+size_t lattice_bytes(const nflow_input& in, int storage);
+// Lay the lattice's buffers out in `mem` (lattice_bytes long) and fill dev.
+void lattice_layout(HostLattice& h, const nflow_input& in, int storage, void* mem);
+// Upload the channel inlet profile (host-computed, double -> float).
+int lattice_upload_inlet(HostLattice& h, hipStream_t st);
+
+class Batch {
+ public:
+  int storage = NFLOW_STORAGE_F32;
+  std::vector<HostLattice> lats;
+  hipStream_t stream = nullptr;
+  int cu_count = 256;
+  nflow_tuning tuning{256, 1, 0, 10, 0.0};
+
+  // device tables
+  LatticeDev* d_lats = nullptr;
+  size_t d_lats_cap = 0;
+  WorkItem* d_items = nullptr;
+  size_t d_items_cap = 0;
+  int nitems = 0;
+  int grid = 0;
+  bool tables_dirty = true;
+
+  int gparity = 0;  // parity argument of the next step launch
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_steps_built = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int last_steps = 0;
+  long long launches = 0;
+
+  // export scratch
+  float* d_aos = nullptr;
+  size_t d_aos_cap = 0;
+  float* d_macro = nullptr;
+  size_t d_macro_cap = 0;
+  double* d_partials = nullptr;
+  size_t d_partials_cap = 0;
+  double* d_out5 = nullptr;
+  int* d_types = nullptr;
+  size_t d_types_cap = 0;
+
+  Batch() = default;
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+  ~Batch();
+
+  int open();  // stream, events, device properties
+  int add_lattice(const nflow_input& in, const std::vector<Tile>& tiles, void* arena_mem);
+  void remove_lattice(int idx);  // frees owned memory; the caller syncs first
+  int upload_tables();
+  int init_lattice(int idx, const float* d_src_aos);
+  int init_all();
+  int launch_steps(int nsteps);  // enqueue only
+  int run(int nsteps);           // enqueue with timing events
+  int synchronize();
+  int timing(double* ms_per_step, int* steps);
+  int export_one(int idx, float* aos, float* rho, float* ux, float* uy, float* uz, double* diag5);
+  int import_one(int idx, const float* aos_host);
+  int node_types(int idx, int* type, int* dirstate);
+  int autotune(int trial_steps, nflow_tuning* out);
+  int set_tuning(const nflow_tuning& t);
+  void destroy_graphs();
+  long long total_nodes() const;
+  size_t device_bytes() const;
+
+ private:
+  int ensure_scratch(size_t aos, size_t macro, size_t partials, size_t types);
+  int build_graph(int parity);
+};
+
+}  // namespace nflow

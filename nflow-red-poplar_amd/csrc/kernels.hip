@@ -1,0 +1,635 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the D3Q19 step.
+//
+// One fused kernel per time step: every thread owns one node, pulls the 19
+// post-collision populations of the previous step from its neighbours
+// (streaming, lbm_codelets.cpp:21-105), applies the wall / inlet / outlet
+// rules in place for links whose donor lies outside the box
+// (lbm_codelets.cpp:325-402, kernel.cu:566-657), forms the macroscopic
+// quantities (lbm_codelets.cpp:248-318), the Smagorinsky relaxation time
+// (lbm_codelets.cpp:558-668) and the SRT or MRT collision
+// (kernel.cu:458-487, lbm_codelets.cpp:409-550), and writes its 19
+// post-collision populations with unit-stride stores.  The reference runs
+// the same maths as five (Poplar) or five-to-seven (CUDA) passes per step.
+//
+// Populations are SoA planes (x fastest) in FP32, or as FP16 / BF16 of
+// f - w_q with FP32 arithmetic.  Per node and step the kernel reads 19 and
+// writes 19 populations: 152 B at FP32, 76 B at FP16/BF16.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "d3q19.hpp"
+#include "lattice.hpp"
+
+namespace nflow {
+using namespace d3q19;
+
+// ---------------------------------------------------------------------------
+// Storage formats
+// ---------------------------------------------------------------------------
+struct StF32 {
+  using T = float;
+  static constexpr bool shifted = false;
+  __device__ static inline float load(const T* __restrict__ p, uint32_t i) { return p[i]; }
+  __device__ static inline void store(T* __restrict__ p, uint32_t i, float v) { p[i] = v; }
+};
+
+struct StF16 {
+  using T = _Float16;
+  static constexpr bool shifted = true;
+  __device__ static inline float load(const T* __restrict__ p, uint32_t i) { return (float)p[i]; }
+  __device__ static inline void store(T* __restrict__ p, uint32_t i, float v) { p[i] = (_Float16)v; }
+};
+
+struct StBF16 {
+  using T = uint16_t;
+  static constexpr bool shifted = true;
+  __device__ static inline float load(const T* __restrict__ p, uint32_t i) {
+    return __uint_as_float((uint32_t)p[i] << 16);
+  }
+  __device__ static inline void store(T* __restrict__ p, uint32_t i, float v) {
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7fffffffu) > 0x7f800000u) {
+      p[i] = (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+    } else {
+      u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+      p[i] = (uint16_t)(u >> 16);
+    }
+  }
+};
+
+template <class St>
+__device__ constexpr float shift_of(int q) {
+  return St::shifted ? W[q] : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Compile-time moment table
+// ---------------------------------------------------------------------------
+struct MTable {
+  int v[Q][Q];
+  constexpr MTable() : v() {
+    for (int k = 0; k < Q; ++k)
+      for (int q = 0; q < Q; ++q) v[k][q] = mrow(k, q);
+  }
+};
+constexpr MTable MT{};
+
+// e_q . (a, b, c) with only the non-zero terms of the integer vector e_q.
+__device__ inline float edot(int q, float a, float b, float c) {
+  float r = 0.0f;
+  bool any = false;
+  if (EX[q] != 0) { r = EX[q] > 0 ? a : -a; any = true; }
+  if (EY[q] != 0) { r = any ? (EY[q] > 0 ? r + b : r - b) : (EY[q] > 0 ? b : -b); any = true; }
+  if (EZ[q] != 0) { r = any ? (EZ[q] > 0 ? r + c : r - c) : (EZ[q] > 0 ? c : -c); any = true; }
+  return r;
+}
+
+__device__ inline int node_type(const LatticeDev& L, int x, int y, int z) {
+  int t = NODE_FLUID;
+  if (L.casetype == CASE_CAVITY) {  // lbm_codelets.cpp:183-201
+    if (y == 0) t = NODE_FIXEDWALL;
+    if (y == L.ny - 1) t = NODE_MOVINGWALL;
+    if (x == 0 || x == L.nx - 1 || z == 0 || z == L.nz - 1) t = NODE_FIXEDWALL;
+  } else {  // square duct: inlet x = 0, outlet x = nx-1, side faces are walls
+    if (x == 0) t = NODE_INLET;
+    if (x == L.nx - 1) t = NODE_OUTLET;
+    if (y == 0 || y == L.ny - 1 || z == 0 || z == L.nz - 1) t = NODE_FIXEDWALL;
+  }
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Streaming + boundary rules: f_q(x) for the step just completed.
+// Donor inside: f_q(x) = f*_q(x - e_q).  Donor outside (the link x -> x + e_opp
+// leaves the box): the rule of the node's type applied to f*_opp(q)(x).
+// ---------------------------------------------------------------------------
+template <class St>
+__device__ inline void gather(const LatticeDev& L, const typename St::T* __restrict__ fin,
+                              const float* __restrict__ side, uint32_t n, int x, int y, int z, float f[Q]) {
+  const bool xlo = x == 0, xhi = x == L.nx - 1;
+  const bool ylo = y == 0, yhi = y == L.ny - 1;
+  const bool zlo = z == 0, zhi = z == L.nz - 1;
+  const uint32_t ps = (uint32_t)L.pstride;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const bool out = (EX[q] > 0 && xlo) || (EX[q] < 0 && xhi) || (EY[q] > 0 && ylo) || (EY[q] < 0 && yhi) ||
+                     (EZ[q] > 0 && zlo) || (EZ[q] < 0 && zhi);
+    const int off = EX[q] + EZ[q] * L.nx + EY[q] * L.nxz;
+    const uint32_t idx = out ? (uint32_t)OPP[q] * ps + n : (uint32_t)q * ps + (uint32_t)((int)n - off);
+    f[q] = St::load(fin, idx) + shift_of<St>(q);
+  }
+  if (xlo | xhi | ylo | yhi | zlo | zhi) {
+    const int t = node_type(L, x, y, z);
+    if (t == NODE_FIXEDWALL) return;  // plain bounce-back: already in place
+    float a = 0.0f, ubx = 0.0f, uby = 0.0f, ubz = 0.0f, usq = 0.0f;
+    if (t == NODE_INLET) {
+      const int yz = y * L.nz + z;
+      a = 6.0f * side[SIDE_RHO_IN * L.ny * L.nz + yz] * L.uin[yz];  // rho(x+1) of the previous macroscopic
+    } else if (t == NODE_OUTLET) {
+      const int yz = y * L.nz + z, pl = L.ny * L.nz;
+      const float* u0 = side + SIDE_U_OUT * pl + yz;
+      const float* u1 = side + SIDE_U_OUT1 * pl + yz;
+      ubx = u0[0] + 0.5f * (u0[0] - u1[0]);
+      uby = u0[pl] + 0.5f * (u0[pl] - u1[pl]);
+      ubz = u0[2 * pl] + 0.5f * (u0[2 * pl] - u1[2 * pl]);
+      usq = ubx * ubx + uby * uby + ubz * ubz;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const bool out = (EX[q] > 0 && xlo) || (EX[q] < 0 && xhi) || (EY[q] > 0 && ylo) || (EY[q] < 0 && yhi) ||
+                       (EZ[q] > 0 && zlo) || (EZ[q] < 0 && zhi);
+      if (!out) continue;
+      if (t == NODE_MOVINGWALL) {  // f_q = f*_opp + 6 w rho0 (e_q . U)
+        f[q] += 6.0f * W[q] * L.refden * edot(q, L.ulid[0], L.ulid[1], L.ulid[2]);
+      } else if (t == NODE_INLET) {  // f_q = f*_opp + 6 w rho(x+1) (e_q . u_in)
+        if (EX[q] != 0) f[q] += (EX[q] > 0 ? W[q] : -W[q]) * a;
+      } else if (t == NODE_OUTLET) {  // anti-bounce-back at rho0
+        const float eu = edot(q, ubx, uby, ubz);
+        f[q] = -f[q] + 2.0f * W[q] * L.refden * (1.0f + 4.5f * eu * eu - 1.5f * usq);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Collision
+// ---------------------------------------------------------------------------
+__device__ inline void moments(const float f[Q], float& rho, float& jx, float& jy, float& jz) {
+  float r = f[0];
+#pragma unroll
+  for (int q = 1; q < Q; ++q) r += f[q];
+  float a = 0.0f, b = 0.0f, c = 0.0f;
+  bool sa = false, sb = false, sc = false;
+#pragma unroll
+  for (int q = 1; q < Q; ++q) {
+    if (EX[q] != 0) { a = sa ? (EX[q] > 0 ? a + f[q] : a - f[q]) : (EX[q] > 0 ? f[q] : -f[q]); sa = true; }
+    if (EY[q] != 0) { b = sb ? (EY[q] > 0 ? b + f[q] : b - f[q]) : (EY[q] > 0 ? f[q] : -f[q]); sb = true; }
+    if (EZ[q] != 0) { c = sc ? (EZ[q] > 0 ? c + f[q] : c - f[q]) : (EZ[q] > 0 ? f[q] : -f[q]); sc = true; }
+  }
+  rho = r;
+  jx = a;
+  jy = b;
+  jz = c;
+}
+
+// Smagorinsky: tau_eddy = -tau/2 + sqrt(tau^2 + 18 sqrt2 Cs^2 sqrt(Q)/rho)/2 with
+// Q = Pi:Pi and Pi = sum_q e e (f - feq) = sum_q e e (f - w rho) - j j / rho.
+__device__ inline float tau_eddy_les(const float f[Q], float rho, float ir, float jx, float jy, float jz, float tau,
+                                     float les_c) {
+  float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pyz = 0.f, pxz = 0.f;
+#pragma unroll
+  for (int q = 1; q < Q; ++q) {
+    const float g = f[q] - W[q] * rho;
+    if (EX[q] != 0) pxx += g;
+    if (EY[q] != 0) pyy += g;
+    if (EZ[q] != 0) pzz += g;
+    if (EX[q] * EY[q] != 0) pxy += (EX[q] * EY[q] > 0) ? g : -g;
+    if (EY[q] * EZ[q] != 0) pyz += (EY[q] * EZ[q] > 0) ? g : -g;
+    if (EX[q] * EZ[q] != 0) pxz += (EX[q] * EZ[q] > 0) ? g : -g;
+  }
+  pxx -= jx * jx * ir;
+  pyy -= jy * jy * ir;
+  pzz -= jz * jz * ir;
+  pxy -= jx * jy * ir;
+  pyz -= jy * jz * ir;
+  pxz -= jx * jz * ir;
+  const float qq = pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pyz * pyz + pxz * pxz);
+  return 0.5f * (__builtin_sqrtf(tau * tau + les_c * __builtin_sqrtf(qq) * ir) - tau);
+}
+
+// BGK, kernel.cu:479-486.
+__device__ inline void collide_srt(float f[Q], float rho, float ir, float jx, float jy, float jz, float omega) {
+  const float ux = jx * ir, uy = jy * ir, uz = jz * ir;
+  const float base = 1.0f - 1.5f * (ux * ux + uy * uy + uz * uz);
+  {
+    const float feq = W[0] * rho * base;
+    f[0] -= omega * (f[0] - feq);
+  }
+#pragma unroll
+  for (int i = 0; i < NPAIR; ++i) {
+    const int a = PA[i], b = PB[i];
+    const float eu = edot(a, ux, uy, uz);
+    const float sym = W[a] * rho * (base + 4.5f * eu * eu);
+    const float asym = W[a] * rho * 3.0f * eu;
+    f[a] -= omega * (f[a] - (sym + asym));
+    f[b] -= omega * (f[b] - (sym - asym));
+  }
+}
+
+// MRT, lbm_codelets.cpp:449-537: f* = f - M^-1 S (M f - meq), with the even /
+// odd split of the orthogonal basis (each row touches only f_a + f_opp(a) or
+// only f_a - f_opp(a)) and M^-1 = M^T diag(1/|row|^2).
+__device__ inline void collide_mrt(float f[Q], float rho, float jx, float jy, float jz, float snu, const LatticeDev& L) {
+  float s[NPAIR], d[NPAIR];
+#pragma unroll
+  for (int i = 0; i < NPAIR; ++i) {
+    s[i] = f[PA[i]] + f[PB[i]];
+    d[i] = f[PA[i]] - f[PB[i]];
+  }
+  // non-conserved moments (rows 1, 2, 4, 6, 8, 9..18)
+  float m[Q];
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    m[k] = 0.0f;
+    if (k == 0 || k == 3 || k == 5 || k == 7) continue;
+    float acc = 0.0f;
+    bool started = false;
+    if (!ODD[k] && MT.v[k][0] != 0) {
+      acc = (float)MT.v[k][0] * f[0];
+      started = true;
+    }
+#pragma unroll
+    for (int i = 0; i < NPAIR; ++i) {
+      const int c = MT.v[k][PA[i]];
+      if (c == 0) continue;
+      const float v = ODD[k] ? d[i] : s[i];
+      if (!started) {
+        acc = (c == 1) ? v : (c == -1 ? -v : (float)c * v);
+        started = true;
+      } else {
+        acc = (c == 1) ? acc + v : (c == -1 ? acc - v : acc + (float)c * v);
+      }
+    }
+    m[k] = acc;
+  }
+  const float ir0 = L.inv_refden;
+  const float jxx = jx * jx, jyy = jy * jy, jzz = jz * jz, jj = jxx + jyy + jzz;
+  float meq[Q];
+#pragma unroll
+  for (int k = 0; k < Q; ++k) meq[k] = 0.0f;
+  meq[1] = -11.0f * rho + 19.0f * jj * ir0;
+  meq[2] = L.weps * rho + L.wepsj * jj * ir0;
+  meq[4] = (-2.0f / 3.0f) * jx;
+  meq[6] = (-2.0f / 3.0f) * jy;
+  meq[8] = (-2.0f / 3.0f) * jz;
+  meq[9] = (2.0f * jxx - (jyy + jzz)) * ir0;
+  meq[10] = L.wxx * meq[9];
+  meq[11] = (jyy - jzz) * ir0;
+  meq[12] = L.wxx * meq[11];
+  meq[13] = jx * jy * ir0;
+  meq[14] = jy * jz * ir0;
+  meq[15] = jx * jz * ir0;
+  // relaxation rates of lbm_codelets.cpp:488-506, folded with 1/|row|^2
+  float dl[Q];
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    float rate;
+    switch (k) {
+      case 1: rate = 1.19f; break;
+      case 2: case 10: case 12: rate = 1.4f; break;
+      case 4: case 6: case 8: rate = 1.2f; break;
+      case 9: case 11: case 13: case 14: case 15: rate = -1.0f; break;
+      case 16: case 17: case 18: rate = 1.98f; break;
+      default: rate = 0.0f;
+    }
+    const float inv_n = 1.0f / (float)MNORM2[k];
+    if (rate == 0.0f)
+      dl[k] = 0.0f;
+    else if (rate < 0.0f)
+      dl[k] = (snu * inv_n) * (m[k] - meq[k]);
+    else
+      dl[k] = (rate * inv_n) * (m[k] - meq[k]);
+  }
+  // back transform: f_a -= E + O, f_opp(a) -= E - O
+  {
+    float e0 = 0.0f;
+    bool started = false;
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+      if (ODD[k] || k == 0 || MT.v[k][0] == 0) continue;
+      const float t = (float)MT.v[k][0] * dl[k];
+      e0 = started ? e0 + t : t;
+      started = true;
+    }
+    f[0] -= e0;
+  }
+#pragma unroll
+  for (int i = 0; i < NPAIR; ++i) {
+    float ev = 0.0f, od = 0.0f;
+    bool se = false, so = false;
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+      if (k == 3 || k == 5 || k == 7) continue;
+      const int c = MT.v[k][PA[i]];
+      if (c == 0) continue;
+      const float t = (c == 1) ? dl[k] : (c == -1 ? -dl[k] : (float)c * dl[k]);
+      if (ODD[k]) {
+        od = so ? od + t : t;
+        so = true;
+      } else {
+        ev = se ? ev + t : t;
+        se = true;
+      }
+    }
+    f[PA[i]] -= ev + od;
+    f[PB[i]] -= ev - od;
+  }
+}
+
+// Macroscopic -> side planes -> LES -> collision -> unit-stride store.
+template <class St>
+__device__ inline void relax_and_store(const LatticeDev& L, float f[Q], typename St::T* __restrict__ fout,
+                                       float* __restrict__ side_out, uint32_t n, int x, int y, int z) {
+  float rho, jx, jy, jz;
+  moments(f, rho, jx, jy, jz);
+  const float ir = 1.0f / rho;
+  if (L.casetype == CASE_CHANNEL) {
+    const int pl = L.ny * L.nz, yz = y * L.nz + z;
+    if (x == 1) side_out[SIDE_RHO_IN * pl + yz] = rho;
+    if (x == L.nx - 1) {
+      side_out[(SIDE_U_OUT + 0) * pl + yz] = jx * ir;
+      side_out[(SIDE_U_OUT + 1) * pl + yz] = jy * ir;
+      side_out[(SIDE_U_OUT + 2) * pl + yz] = jz * ir;
+    }
+    if (x == L.nx - 2) {
+      side_out[(SIDE_U_OUT1 + 0) * pl + yz] = jx * ir;
+      side_out[(SIDE_U_OUT1 + 1) * pl + yz] = jy * ir;
+      side_out[(SIDE_U_OUT1 + 2) * pl + yz] = jz * ir;
+    }
+  }
+  float te = 0.0f;
+  if (L.les) te = tau_eddy_les(f, rho, ir, jx, jy, jz, L.tau, L.les_c);
+  if (L.model == MODEL_MRT) {
+    collide_mrt(f, rho, jx, jy, jz, 1.0f / (L.tau + te), L);
+  } else {
+    collide_srt(f, rho, ir, jx, jy, jz, 1.0f / (L.tau + te));
+  }
+  const uint32_t ps = (uint32_t)L.pstride;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) St::store(fout, (uint32_t)q * ps + n, f[q] - shift_of<St>(q));
+}
+
+__device__ inline void coords(const LatticeDev& L, uint32_t n, int& x, int& y, int& z) {
+  const uint32_t yy = fastdiv(n, L.div_nxz);
+  const uint32_t r = n - yy * (uint32_t)L.nxz;
+  const uint32_t zz = fastdiv(r, L.div_nx);
+  x = (int)(r - zz * (uint32_t)L.nx);
+  y = (int)yy;
+  z = (int)zz;
+}
+
+// ---------------------------------------------------------------------------
+// The time step: one launch advances every lattice in the work list.
+// ---------------------------------------------------------------------------
+template <class St, int BS>
+__global__ void __launch_bounds__(BS) k_step(const LatticeDev* __restrict__ lats, const WorkItem* __restrict__ items,
+                                             int nitems, int parity) {
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem wi = items[it];
+    if (threadIdx.x < wi.count) {
+      const LatticeDev& L = lats[wi.lat];
+      const uint32_t n = wi.start + threadIdx.x;
+      const int p = parity ^ L.parity0;
+      int x, y, z;
+      coords(L, n, x, y, z);
+      float f[Q];
+      gather<St>(L, static_cast<const typename St::T*>(L.buf[p]), L.side[p], n, x, y, z, f);
+      relax_and_store<St>(L, f, static_cast<typename St::T*>(L.buf[p ^ 1]), L.side[p ^ 1], n, x, y, z);
+    }
+  }
+}
+
+// Initialisation (Initialize vertex, lbm_codelets.cpp:140-241) or import of
+// an arF array: the streamed state goes to buf[cur ^ 1] (what an export at
+// step 0 returns), its collision to buf[cur] (what the first step streams).
+template <class St>
+__global__ void __launch_bounds__(256) k_init(LatticeDev L, int cur, const float* __restrict__ src) {
+  const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+  if (n >= L.n) return;
+  int x, y, z;
+  coords(L, n, x, y, z);
+  float f[Q];
+  if (src) {
+    const size_t b = (((size_t)(y + 1) * (L.nz + 2) + (z + 1)) * (L.nx + 2) + (x + 1)) * Q;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = src[b + q];
+  } else {
+    float ux = 0.f, uy = 0.f, uz = 0.f;
+    if (L.init_lid && node_type(L, x, y, z) == NODE_MOVINGWALL) {
+      ux = L.ulid[0];
+      uy = L.ulid[1];
+      uz = L.ulid[2];
+    }
+    const float usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const float eu = edot(q, ux, uy, uz);
+      f[q] = W[q] * L.refden * (1.0f + 3.0f * eu + 4.5f * eu * eu - 1.5f * usq);
+    }
+  }
+  typename St::T* streamed = static_cast<typename St::T*>(L.buf[cur ^ 1]);
+  const uint32_t ps = (uint32_t)L.pstride;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) St::store(streamed, (uint32_t)q * ps + n, f[q] - shift_of<St>(q));
+  relax_and_store<St>(L, f, static_cast<typename St::T*>(L.buf[cur]), L.side[cur], n, x, y, z);
+}
+
+// ---------------------------------------------------------------------------
+// Export: the streamed state of the last completed step (stream + boundary
+// rules applied to buf[prev]), written as padded AoS arF and/or rho/u, and
+// per-workgroup FP64 partial sums of mass, momentum and |rho - rho_prev|.
+// ---------------------------------------------------------------------------
+constexpr int EXPORT_BS = 256;
+
+__device__ inline double block_sum(double v, double* sh) {
+  // deterministic: fixed butterfly within the wave, fixed order across waves
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < EXPORT_BS / 64; ++w) r += sh[w];
+  return r;
+}
+
+template <class St>
+__global__ void __launch_bounds__(EXPORT_BS) k_export(LatticeDev L, int prev, int prev_streamed, float* __restrict__ aos,
+                                                      float* __restrict__ rho_o, float* __restrict__ ux_o,
+                                                      float* __restrict__ uy_o, float* __restrict__ uz_o,
+                                                      double* __restrict__ partials) {
+  __shared__ double sh[EXPORT_BS / 64];
+  const uint32_t n = blockIdx.x * (uint32_t)EXPORT_BS + threadIdx.x;
+  double dm = 0.0, dx = 0.0, dy = 0.0, dz = 0.0, dr = 0.0;
+  if (n < L.n) {
+    int x, y, z;
+    coords(L, n, x, y, z);
+    const typename St::T* pb = static_cast<const typename St::T*>(L.buf[prev]);
+    const uint32_t ps = (uint32_t)L.pstride;
+    float f[Q];
+    if (prev_streamed) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) f[q] = St::load(pb, (uint32_t)q * ps + n) + shift_of<St>(q);
+    } else {
+      gather<St>(L, pb, L.side[prev], n, x, y, z, f);
+    }
+    float rho, jx, jy, jz;
+    moments(f, rho, jx, jy, jz);
+    const float ir = 1.0f / rho;
+    if (aos) {
+      const size_t b = (((size_t)(y + 1) * (L.nz + 2) + (z + 1)) * (L.nx + 2) + (x + 1)) * Q;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) aos[b + q] = f[q];
+    }
+    if (rho_o) {
+      rho_o[n] = rho;
+      ux_o[n] = jx * ir;
+      uy_o[n] = jy * ir;
+      uz_o[n] = jz * ir;
+    }
+    if (partials) {
+      double rn = 0.0, ro = 0.0;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) rn += (double)f[q];
+      if (!prev_streamed) {
+        // the collision conserves mass, so sum_q f*_q(prev) is the previous step's density
+#pragma unroll
+        for (int q = 0; q < Q; ++q) ro += (double)(St::load(pb, (uint32_t)q * ps + n) + shift_of<St>(q));
+      }
+      dm = rn;
+      dx = (double)jx;
+      dy = (double)jy;
+      dz = (double)jz;
+      dr = fabs(rn - ro);
+    }
+  }
+  if (partials) {
+    const double a = block_sum(dm, sh);
+    const double b = block_sum(dx, sh);
+    const double c = block_sum(dy, sh);
+    const double d = block_sum(dz, sh);
+    const double e = block_sum(dr, sh);
+    if (threadIdx.x == 0) {
+      double* o = partials + (size_t)blockIdx.x * 5;
+      o[0] = a;
+      o[1] = b;
+      o[2] = c;
+      o[3] = d;
+      o[4] = e;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_reduce(const double* __restrict__ partials, int nblocks,
+                                                double* __restrict__ out) {
+  __shared__ double sh[256];
+  for (int c = 0; c < 5; ++c) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += 256) v += partials[(size_t)i * 5 + c];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0];
+    __syncthreads();
+  }
+}
+
+// keSetNodeType + keInitialNodeDirection (kernel.cu:1020-1097) for read-back.
+__global__ void __launch_bounds__(256) k_node_type(LatticeDev L, int* __restrict__ type, int* __restrict__ dirstate) {
+  const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+  if (n >= L.n) return;
+  int x, y, z;
+  coords(L, n, x, y, z);
+  int ds = 1;
+#pragma unroll
+  for (int q = 1; q < Q; ++q) {
+    const int x1 = x + EX[q], y1 = y + EY[q], z1 = z + EZ[q];
+    if (x1 >= 0 && x1 < L.nx && y1 >= 0 && y1 < L.ny && z1 >= 0 && z1 < L.nz) ds |= 1 << q;
+  }
+  type[n] = node_type(L, x, y, z);
+  dirstate[n] = ds;
+}
+
+// Streaming-copy ceiling: `streams` arrays of `per` floats, unit stride.
+__global__ void __launch_bounds__(256) k_copy_streams(const float* __restrict__ src, float* __restrict__ dst,
+                                                      size_t per, int streams) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < per; i += stride)
+    for (int s = 0; s < streams; ++s) dst[(size_t)s * per + i] = src[(size_t)s * per + i];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+template <class St>
+static hipError_t step_bs(int block, int grid, hipStream_t st, const LatticeDev* d_lats, const WorkItem* d_items,
+                          int nitems, int parity) {
+  switch (block) {
+    case 64: hipLaunchKernelGGL((k_step<St, 64>), dim3(grid), dim3(64), 0, st, d_lats, d_items, nitems, parity); break;
+    case 128: hipLaunchKernelGGL((k_step<St, 128>), dim3(grid), dim3(128), 0, st, d_lats, d_items, nitems, parity); break;
+    case 256: hipLaunchKernelGGL((k_step<St, 256>), dim3(grid), dim3(256), 0, st, d_lats, d_items, nitems, parity); break;
+    case 512: hipLaunchKernelGGL((k_step<St, 512>), dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_step(int storage, int block, int grid, hipStream_t st, const LatticeDev* d_lats,
+                       const WorkItem* d_items, int nitems, int parity) {
+  if (grid <= 0 || nitems <= 0) return hipSuccess;
+  switch (storage) {
+    case ST_F32: return step_bs<StF32>(block, grid, st, d_lats, d_items, nitems, parity);
+    case ST_F16: return step_bs<StF16>(block, grid, st, d_lats, d_items, nitems, parity);
+    case ST_BF16: return step_bs<StBF16>(block, grid, st, d_lats, d_items, nitems, parity);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_init(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src) {
+  const int grid = (int)((L.n + 255u) / 256u);
+  switch (storage) {
+    case ST_F32: hipLaunchKernelGGL(k_init<StF32>, dim3(grid), dim3(256), 0, st, L, cur, src); break;
+    case ST_F16: hipLaunchKernelGGL(k_init<StF16>, dim3(grid), dim3(256), 0, st, L, cur, src); break;
+    case ST_BF16: hipLaunchKernelGGL(k_init<StBF16>, dim3(grid), dim3(256), 0, st, L, cur, src); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int export_block_size() { return EXPORT_BS; }
+
+hipError_t launch_export(int storage, hipStream_t st, const LatticeDev& L, int prev, int prev_streamed, float* aos,
+                         float* rho, float* ux, float* uy, float* uz, double* partials, int* nblocks_out) {
+  const int grid = (int)((L.n + EXPORT_BS - 1) / EXPORT_BS);
+  if (nblocks_out) *nblocks_out = grid;
+  switch (storage) {
+    case ST_F32:
+      hipLaunchKernelGGL(k_export<StF32>, dim3(grid), dim3(EXPORT_BS), 0, st, L, prev, prev_streamed, aos, rho, ux, uy,
+                         uz, partials);
+      break;
+    case ST_F16:
+      hipLaunchKernelGGL(k_export<StF16>, dim3(grid), dim3(EXPORT_BS), 0, st, L, prev, prev_streamed, aos, rho, ux, uy,
+                         uz, partials);
+      break;
+    case ST_BF16:
+      hipLaunchKernelGGL(k_export<StBF16>, dim3(grid), dim3(EXPORT_BS), 0, st, L, prev, prev_streamed, aos, rho, ux,
+                         uy, uz, partials);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(hipStream_t st, const double* partials, int nblocks, double* out5) {
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, partials, nblocks, out5);
+  return hipGetLastError();
+}
+
+hipError_t launch_node_type(hipStream_t st, const LatticeDev& L, int* type, int* dirstate) {
+  const int grid = (int)((L.n + 255u) / 256u);
+  hipLaunchKernelGGL(k_node_type, dim3(grid), dim3(256), 0, st, L, type, dirstate);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_streams(hipStream_t st, const float* src, float* dst, size_t per, int streams) {
+  size_t blocks = (per + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_copy_streams, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, per, streams);
+  return hipGetLastError();
+}
+
+}  // namespace nflow

@@ -1,0 +1,86 @@
+// lattice.hpp -- structures shared by the host runtime and the HIP kernels.
+#pragma once
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+namespace nflow {
+
+enum : int { CASE_CHANNEL = 0, CASE_CAVITY = 1 };
+enum : int { MODEL_SRT = 0, MODEL_MRT = 1 };
+enum : int { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2 };
+enum : int {
+  NODE_FLUID = 1 << 1,
+  NODE_FIXEDWALL = 1 << 2,
+  NODE_MOVINGWALL = 1 << 3,
+  NODE_INLET = 1 << 4,
+  NODE_OUTLET = 1 << 5,
+  NODE_STRUCT = 1 << 6
+};
+
+// Channel boundary planes kept between steps, each [ny][nz]: the density of
+// the x = 1 plane (used by the inlet, kernel.cu:596-597) and the velocity of
+// the x = nx-1 and x = nx-2 planes (used by the outlet, kernel.cu:645-648).
+enum : int { SIDE_RHO_IN = 0, SIDE_U_OUT = 1, SIDE_U_OUT1 = 4, SIDE_PLANES = 7 };
+
+// Exact unsigned division by a runtime constant d for n < 2^31:
+// floor(n / d) = (n * magic) >> shift with shift = 31 + ceil(log2 d) and
+// magic = ceil(2^shift / d) < 2^32 (Granlund & Montgomery 1994, Thm 4.2).
+struct FastDiv {
+  uint32_t magic;
+  uint32_t shift;
+};
+
+__host__ __device__ inline uint32_t fastdiv(uint32_t n, FastDiv d) {
+  return (uint32_t)(((uint64_t)n * d.magic) >> d.shift);
+}
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  const uint32_t shift = 31 + l;
+  const uint64_t p = 1ull << shift;
+  FastDiv r;
+  r.magic = (uint32_t)((p + d - 1) / d);
+  r.shift = shift;
+  return r;
+}
+
+// One lattice as the kernels see it.  Populations are SoA planes
+// buf[i][q * pstride + n] with n = x + nx*(z + nz*y); the two buffers
+// alternate each step (buffer parity ^ parity0 is the one read).
+struct alignas(16) LatticeDev {
+  void* buf[2];
+  float* side[2];
+  const float* uin;   // channel inlet profile [ny][nz]
+  long long pstride;  // elements between population planes
+  int nx, ny, nz, nxz;
+  uint32_t n;         // interior nodes
+  FastDiv div_nx, div_nxz;
+  int casetype, model, les, parity0;
+  int init_lid, pad0, pad1, pad2;
+  float tau, les_c, refden, inv_refden;
+  float weps, wepsj, wxx, pad3;
+  float ulid[4];
+};
+
+// A run of consecutive node ids of one lattice, processed by one workgroup.
+struct alignas(16) WorkItem {
+  int lat;
+  uint32_t start;
+  uint32_t count;
+  uint32_t pad;
+};
+
+// Kernel launchers (kernels.hip).
+hipError_t launch_step(int storage, int block, int grid, hipStream_t st, const LatticeDev* d_lats,
+                       const WorkItem* d_items, int nitems, int parity);
+hipError_t launch_init(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src_aos);
+hipError_t launch_export(int storage, hipStream_t st, const LatticeDev& L, int prev, int prev_streamed, float* aos,
+                         float* rho, float* ux, float* uy, float* uz, double* partials, int* nblocks_out);
+hipError_t launch_reduce(hipStream_t st, const double* partials, int nblocks, double* out5);
+hipError_t launch_node_type(hipStream_t st, const LatticeDev& L, int* type, int* dirstate);
+hipError_t launch_copy_streams(hipStream_t st, const float* src, float* dst, size_t per_stream, int streams);
+int export_block_size();
+
+}  // namespace nflow

@@ -1,0 +1,771 @@
+// runtime.hip -- host runtime behind the C ABI: device memory of each
+// lattice, the work list, graph-captured stepping, the autotuner and the
+// export / diagnostics path.  Replaces the Poplar graph build + Engine of
+// main.cpp:187-317 and the CUDA host loop of kernel.cu:18-280.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "batch.hpp"
+#include "d3q19.hpp"
+#include "errors.hpp"
+#include "nflow.h"
+
+namespace nflow {
+
+static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+static size_t elem_size(int storage) { return storage == NFLOW_STORAGE_F32 ? 4 : 2; }
+
+int validate_input(const nflow_input& in, int storage) {
+  if (in.nx < 1 || in.ny < 1 || in.nz < 1) return fail(NFLOW_EINVAL, "lattice extents must be >= 1");
+  if (in.casetype != NFLOW_CAVITY && in.casetype != NFLOW_CHANNEL)
+    return fail(NFLOW_EINVAL, "casetype must be NFLOW_CAVITY or NFLOW_CHANNEL");
+  if (in.casetype == NFLOW_CHANNEL && in.nx < 3) return fail(NFLOW_EINVAL, "channel needs nx >= 3");
+  if (storage < NFLOW_STORAGE_F32 || storage > NFLOW_STORAGE_BF16) return fail(NFLOW_EINVAL, "bad storage");
+  const unsigned long long n = (unsigned long long)in.nx * in.ny * in.nz;
+  const unsigned long long pstride = round_up(n, 128);
+  if (pstride * 19ull >= (1ull << 32)) return fail(NFLOW_EINVAL, "lattice of %llu nodes exceeds the 32-bit index range", n);
+  if (!(in.tau > 0.5)) return fail(NFLOW_EINVAL, "tau must exceed 0.5 (got %g)", in.tau);
+  if (!(in.ref_den > 0.0)) return fail(NFLOW_EINVAL, "ref_den must be positive");
+  if (in.enable_les && in.smagorinsky < 0.0) return fail(NFLOW_EINVAL, "smagorinsky must be >= 0");
+  return 0;
+}
+
+struct Layout {
+  size_t pstride, pop, side, uin, total;
+};
+
+static Layout layout_of(const nflow_input& in, int storage) {
+  Layout l;
+  const size_t n = (size_t)in.nx * in.ny * in.nz;
+  l.pstride = round_up(n, 128);
+  l.pop = round_up(19 * l.pstride * elem_size(storage), 256);
+  l.side = round_up((size_t)SIDE_PLANES * in.ny * in.nz * sizeof(float), 256);
+  l.uin = round_up((size_t)in.ny * in.nz * sizeof(float), 256);
+  l.total = 2 * l.pop + 2 * l.side + l.uin;
+  return l;
+}
+
+size_t lattice_bytes(const nflow_input& in, int storage) { return layout_of(in, storage).total; }
+
+void lattice_layout(HostLattice& h, const nflow_input& in, int storage, void* mem) {
+  const Layout l = layout_of(in, storage);
+  char* p = static_cast<char*>(mem);
+  h.in = in;
+  h.in.storage = storage;
+  h.mem = mem;
+  h.bytes = l.total;
+  LatticeDev& d = h.dev;
+  std::memset(&d, 0, sizeof(d));
+  d.buf[0] = p;
+  d.buf[1] = p + l.pop;
+  d.side[0] = reinterpret_cast<float*>(p + 2 * l.pop);
+  d.side[1] = reinterpret_cast<float*>(p + 2 * l.pop + l.side);
+  d.uin = reinterpret_cast<const float*>(p + 2 * l.pop + 2 * l.side);
+  d.pstride = (long long)l.pstride;
+  d.nx = in.nx;
+  d.ny = in.ny;
+  d.nz = in.nz;
+  d.nxz = in.nx * in.nz;
+  d.n = (uint32_t)((size_t)in.nx * in.ny * in.nz);
+  d.div_nx = make_fastdiv((uint32_t)in.nx);
+  d.div_nxz = make_fastdiv((uint32_t)d.nxz);
+  d.casetype = in.casetype == NFLOW_CAVITY ? CASE_CAVITY : CASE_CHANNEL;
+  d.model = in.enable_mrt ? MODEL_MRT : MODEL_SRT;
+  // with Cs = 0 the Smagorinsky term vanishes identically (tau_eddy = 0)
+  d.les = (in.enable_les && in.smagorinsky > 0.0) ? 1 : 0;
+  d.parity0 = 0;
+  d.init_lid = in.init_lid_velocity ? 1 : 0;
+  d.tau = (float)in.tau;
+  d.les_c = (float)(18.0 * 1.41421356237 * in.smagorinsky * in.smagorinsky);
+  d.refden = (float)in.ref_den;
+  d.inv_refden = (float)(1.0 / in.ref_den);
+  d.weps = (float)in.w_eps;
+  d.wepsj = (float)in.w_epsj;
+  d.wxx = (float)in.w_xx;
+  d.ulid[0] = (float)in.ref_vel[0];
+  d.ulid[1] = (float)in.ref_vel[1];
+  d.ulid[2] = (float)in.ref_vel[2];
+}
+
+// splitmix64 (Steele, Lea & Flood 2014), counter-based so the inlet
+// perturbation of node (y, z) depends only on (seed, y, z).
+static uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+int lattice_upload_inlet(HostLattice& h, hipStream_t st) {
+  const nflow_input& in = h.in;
+  if (in.casetype != NFLOW_CHANNEL) return 0;
+  std::vector<float> u((size_t)in.ny * in.nz);
+  for (int y = 0; y < in.ny; ++y)
+    for (int z = 0; z < in.nz; ++z) {
+      const double eta = (y + 0.5) / in.ny, zeta = (z + 0.5) / in.nz;
+      const uint64_t key = (uint64_t)in.seed ^ (((uint64_t)(uint32_t)y << 32) | (uint64_t)(uint32_t)z);
+      const double xi = 2.0 * ((double)(splitmix64(key) >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+      u[(size_t)y * in.nz + z] =
+          (float)(in.inlet_peak * 16.0 * eta * (1.0 - eta) * zeta * (1.0 - zeta) * (1.0 + in.perturb * xi));
+    }
+  NFLOW_HIP(hipMemcpyAsync(const_cast<float*>(h.dev.uin), u.data(), u.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  NFLOW_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+Batch::~Batch() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  destroy_graphs();
+  for (auto& h : lats)
+    if (h.own_mem && h.mem) (void)hipFree(h.mem);
+  (void)hipFree(d_lats);
+  (void)hipFree(d_items);
+  (void)hipFree(d_aos);
+  (void)hipFree(d_macro);
+  (void)hipFree(d_partials);
+  (void)hipFree(d_out5);
+  (void)hipFree(d_types);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Batch::open() {
+  int dev = 0;
+  NFLOW_HIP(hipGetDevice(&dev));
+  NFLOW_HIP(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev));
+  NFLOW_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  NFLOW_HIP(hipEventCreate(&ev0));
+  NFLOW_HIP(hipEventCreate(&ev1));
+  return 0;
+}
+
+int Batch::add_lattice(const nflow_input& in, const std::vector<Tile>& tiles, void* arena_mem) {
+  int rc = validate_input(in, storage);
+  if (rc) return rc;
+  HostLattice h;
+  void* mem = arena_mem;
+  const size_t bytes = lattice_bytes(in, storage);
+  if (!mem) {
+    if (hipMalloc(&mem, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(NFLOW_ENOMEM, "hipMalloc of %zu bytes for a %dx%dx%d lattice failed", bytes, in.nx, in.ny, in.nz);
+    }
+    h.own_mem = true;
+  }
+  lattice_layout(h, in, storage, mem);
+  h.tiles = tiles;
+  if (h.tiles.empty()) h.tiles.push_back(Tile{0, in.nx, 0, in.ny, 0, in.nz});
+  rc = lattice_upload_inlet(h, stream);
+  if (rc) {
+    if (h.own_mem) (void)hipFree(mem);
+    return rc;
+  }
+  lats.push_back(h);
+  tables_dirty = true;
+  return 0;
+}
+
+void Batch::remove_lattice(int idx) {
+  if (idx < 0 || idx >= (int)lats.size()) return;
+  if (lats[idx].own_mem && lats[idx].mem) (void)hipFree(lats[idx].mem);
+  lats.erase(lats.begin() + idx);
+  tables_dirty = true;
+}
+
+void Batch::destroy_graphs() {
+  for (auto& g : graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  graph_steps_built = 0;
+}
+
+long long Batch::total_nodes() const {
+  long long s = 0;
+  for (const auto& h : lats) s += (long long)h.dev.n;
+  return s;
+}
+
+size_t Batch::device_bytes() const {
+  size_t s = 0;
+  for (const auto& h : lats) s += h.bytes;
+  return s;
+}
+
+int Batch::set_tuning(const nflow_tuning& t) {
+  if (t.block != 64 && t.block != 128 && t.block != 256 && t.block != 512)
+    return fail(NFLOW_EINVAL, "tuning.block must be 64, 128, 256 or 512");
+  if (t.persistent < 0 || t.graph_steps < 0 || (t.graph_steps & 1))
+    return fail(NFLOW_EINVAL, "tuning.persistent must be >= 0 and graph_steps even and >= 0");
+  tuning = t;
+  tables_dirty = true;
+  destroy_graphs();
+  return 0;
+}
+
+// Work list: each tile's nodes as runs of consecutive node ids (whole slabs
+// merge into one run), cut into workgroup-sized items.  With xcd_order the
+// items are dealt so that the workgroups of XCD k (dispatch id = k mod 8)
+// sweep one contiguous eighth of the list, keeping the y/z neighbour planes
+// of a workgroup in the same XCD's L2.
+int Batch::upload_tables() {
+  if (!tables_dirty) return 0;
+  NFLOW_HIP(hipStreamSynchronize(stream));
+  destroy_graphs();
+  const int bs = tuning.block;
+  std::vector<WorkItem> items;
+  std::vector<LatticeDev> devs(lats.size());
+  for (size_t li = 0; li < lats.size(); ++li) {
+    const HostLattice& h = lats[li];
+    devs[li] = h.dev;
+    const uint32_t nx = (uint32_t)h.in.nx, nxz = (uint32_t)h.in.nx * (uint32_t)h.in.nz;
+    auto emit = [&](uint32_t a, uint32_t b) {
+      for (uint32_t s = a; s < b; s += (uint32_t)bs)
+        items.push_back(WorkItem{(int)li, s, std::min<uint32_t>((uint32_t)bs, b - s), 0u});
+    };
+    for (const Tile& t : h.tiles) {
+      if (t.x0 == 0 && t.x1 == h.in.nx && t.z0 == 0 && t.z1 == h.in.nz) {
+        emit((uint32_t)t.y0 * nxz, (uint32_t)t.y1 * nxz);
+      } else if (t.x0 == 0 && t.x1 == h.in.nx) {
+        for (int y = t.y0; y < t.y1; ++y) emit(y * nxz + t.z0 * nx, y * nxz + t.z1 * nx);
+      } else {
+        for (int y = t.y0; y < t.y1; ++y)
+          for (int z = t.z0; z < t.z1; ++z) emit(y * nxz + z * nx + t.x0, y * nxz + z * nx + t.x1);
+      }
+    }
+  }
+  if (tuning.xcd_order && items.size() > 8) {
+    const size_t m = items.size();
+    std::vector<std::vector<WorkItem>> chunk(8);
+    for (int k = 0; k < 8; ++k) chunk[k].assign(items.begin() + m * k / 8, items.begin() + m * (k + 1) / 8);
+    size_t maxlen = 0;
+    for (auto& c : chunk) maxlen = std::max(maxlen, c.size());
+    std::vector<WorkItem> dealt(8 * maxlen, WorkItem{0, 0u, 0u, 0u});
+    for (int k = 0; k < 8; ++k)
+      for (size_t j = 0; j < chunk[k].size(); ++j) dealt[8 * j + k] = chunk[k][j];
+    items.swap(dealt);
+  }
+  nitems = (int)items.size();
+  if (tuning.persistent > 0) {
+    grid = tuning.persistent * cu_count;
+    grid = (grid + 7) / 8 * 8;
+    grid = std::min(grid, std::max(nitems, 1));
+  } else {
+    grid = nitems;
+  }
+  if (devs.size() > d_lats_cap) {
+    (void)hipFree(d_lats);
+    d_lats = nullptr;
+    NFLOW_HIP(hipMalloc(&d_lats, devs.size() * sizeof(LatticeDev)));
+    d_lats_cap = devs.size();
+  }
+  if (items.size() > d_items_cap) {
+    (void)hipFree(d_items);
+    d_items = nullptr;
+    NFLOW_HIP(hipMalloc(&d_items, items.size() * sizeof(WorkItem)));
+    d_items_cap = items.size();
+  }
+  if (!devs.empty())
+    NFLOW_HIP(hipMemcpy(d_lats, devs.data(), devs.size() * sizeof(LatticeDev), hipMemcpyHostToDevice));
+  if (!items.empty())
+    NFLOW_HIP(hipMemcpy(d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+  tables_dirty = false;
+  return 0;
+}
+
+int Batch::init_lattice(int idx, const float* d_src) {
+  HostLattice& h = lats[idx];
+  NFLOW_HIP(launch_init(storage, stream, h.dev, gparity, d_src));
+  h.cur = gparity;
+  h.steps_done = 0;
+  h.initialized = true;
+  return 0;
+}
+
+int Batch::init_all() {
+  for (size_t i = 0; i < lats.size(); ++i) {
+    int rc = init_lattice((int)i, nullptr);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int Batch::build_graph(int parity) {
+  hipGraph_t g = nullptr;
+  NFLOW_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  int p = parity;
+  hipError_t err = hipSuccess;
+  for (int s = 0; s < tuning.graph_steps && err == hipSuccess; ++s) {
+    err = launch_step(storage, tuning.block, grid, stream, d_lats, d_items, nitems, p);
+    p ^= 1;
+  }
+  hipError_t e2 = hipStreamEndCapture(stream, &g);
+  if (err != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(NFLOW_EDEVICE, "graph capture failed: %s", hipGetErrorString(err != hipSuccess ? err : e2));
+  }
+  hipError_t e3 = hipGraphInstantiate(&graph[parity], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e3 != hipSuccess) return fail(NFLOW_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+  graph_steps_built = tuning.graph_steps;
+  return 0;
+}
+
+int Batch::launch_steps(int nsteps) {
+  if (nsteps < 0) return fail(NFLOW_EINVAL, "nsteps must be >= 0");
+  for (const auto& h : lats)
+    if (!h.initialized) return fail(NFLOW_ESTATE, "run before initialize");
+  int rc = upload_tables();
+  if (rc) return rc;
+  int left = nsteps;
+  const int gs = tuning.graph_steps;
+  if (gs > 0 && left >= gs && nitems > 0) {
+    if (graph_steps_built != gs) destroy_graphs();
+    if (!graph[gparity]) {
+      rc = build_graph(gparity);
+      if (rc) return rc;
+    }
+    const int reps = left / gs;
+    for (int r = 0; r < reps; ++r) NFLOW_HIP(hipGraphLaunch(graph[gparity], stream));
+    left -= reps * gs;
+    launches += (long long)reps * gs;
+  }
+  for (int s = 0; s < left; ++s) {
+    NFLOW_HIP(launch_step(storage, tuning.block, grid, stream, d_lats, d_items, nitems, gparity));
+    gparity ^= 1;
+    ++launches;
+  }
+  for (auto& h : lats) {
+    h.steps_done += nsteps;
+    h.cur = gparity;
+  }
+  return 0;
+}
+
+int Batch::run(int nsteps) {
+  NFLOW_HIP(hipEventRecord(ev0, stream));
+  int rc = launch_steps(nsteps);
+  if (rc) return rc;
+  NFLOW_HIP(hipEventRecord(ev1, stream));
+  last_steps = nsteps;
+  return 0;
+}
+
+int Batch::synchronize() {
+  NFLOW_HIP(hipStreamSynchronize(stream));
+  return 0;
+}
+
+int Batch::timing(double* ms_per_step, int* steps) {
+  NFLOW_HIP(hipEventSynchronize(ev1));
+  float ms = 0.0f;
+  NFLOW_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  if (ms_per_step) *ms_per_step = last_steps > 0 ? (double)ms / last_steps : 0.0;
+  if (steps) *steps = last_steps;
+  return 0;
+}
+
+int Batch::ensure_scratch(size_t aos, size_t macro, size_t partials, size_t types) {
+  NFLOW_HIP(hipStreamSynchronize(stream));
+  if (aos > d_aos_cap) {
+    (void)hipFree(d_aos);
+    d_aos = nullptr;
+    if (hipMalloc(&d_aos, aos * sizeof(float)) != hipSuccess) return fail(NFLOW_ENOMEM, "export scratch");
+    d_aos_cap = aos;
+  }
+  if (macro > d_macro_cap) {
+    (void)hipFree(d_macro);
+    d_macro = nullptr;
+    if (hipMalloc(&d_macro, macro * sizeof(float)) != hipSuccess) return fail(NFLOW_ENOMEM, "export scratch");
+    d_macro_cap = macro;
+  }
+  if (partials > d_partials_cap) {
+    (void)hipFree(d_partials);
+    d_partials = nullptr;
+    if (hipMalloc(&d_partials, partials * sizeof(double)) != hipSuccess) return fail(NFLOW_ENOMEM, "export scratch");
+    d_partials_cap = partials;
+  }
+  if (!d_out5) NFLOW_HIP(hipMalloc(&d_out5, 8 * sizeof(double)));
+  if (types > d_types_cap) {
+    (void)hipFree(d_types);
+    d_types = nullptr;
+    if (hipMalloc(&d_types, types * sizeof(int)) != hipSuccess) return fail(NFLOW_ENOMEM, "export scratch");
+    d_types_cap = types;
+  }
+  return 0;
+}
+
+static size_t padded_count(const nflow_input& in) {
+  return (size_t)(in.nx + 2) * (in.ny + 2) * (in.nz + 2) * 19;
+}
+
+int Batch::export_one(int idx, float* aos, float* rho, float* ux, float* uy, float* uz, double* diag5) {
+  if (idx < 0 || idx >= (int)lats.size()) return fail(NFLOW_EINVAL, "lattice index out of range");
+  HostLattice& h = lats[idx];
+  if (!h.initialized) return fail(NFLOW_ESTATE, "read before initialize");
+  const size_t n = h.dev.n;
+  const size_t nb = (n + export_block_size() - 1) / export_block_size();
+  int rc = ensure_scratch(aos ? padded_count(h.in) : 0, rho ? 4 * n : 0, diag5 ? 5 * nb : 0, 0);
+  if (rc) return rc;
+  if (aos) NFLOW_HIP(hipMemsetAsync(d_aos, 0, padded_count(h.in) * sizeof(float), stream));
+  const int prev = h.cur ^ 1;
+  const int prev_streamed = h.steps_done == 0 ? 1 : 0;
+  int nblocks = 0;
+  float* m = rho ? d_macro : nullptr;
+  NFLOW_HIP(launch_export(storage, stream, h.dev, prev, prev_streamed, aos ? d_aos : nullptr, m, m ? m + n : nullptr,
+                          m ? m + 2 * n : nullptr, m ? m + 3 * n : nullptr, diag5 ? d_partials : nullptr, &nblocks));
+  if (diag5) NFLOW_HIP(launch_reduce(stream, d_partials, nblocks, d_out5));
+  if (aos) NFLOW_HIP(hipMemcpyAsync(aos, d_aos, padded_count(h.in) * sizeof(float), hipMemcpyDeviceToHost, stream));
+  if (rho) {
+    NFLOW_HIP(hipMemcpyAsync(rho, m, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    NFLOW_HIP(hipMemcpyAsync(ux, m + n, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    NFLOW_HIP(hipMemcpyAsync(uy, m + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    NFLOW_HIP(hipMemcpyAsync(uz, m + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+  }
+  if (diag5) NFLOW_HIP(hipMemcpyAsync(diag5, d_out5, 5 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  NFLOW_HIP(hipStreamSynchronize(stream));
+  return 0;
+}
+
+int Batch::import_one(int idx, const float* aos_host) {
+  if (idx < 0 || idx >= (int)lats.size()) return fail(NFLOW_EINVAL, "lattice index out of range");
+  HostLattice& h = lats[idx];
+  int rc = ensure_scratch(padded_count(h.in), 0, 0, 0);
+  if (rc) return rc;
+  NFLOW_HIP(hipMemcpyAsync(d_aos, aos_host, padded_count(h.in) * sizeof(float), hipMemcpyHostToDevice, stream));
+  rc = init_lattice(idx, d_aos);
+  if (rc) return rc;
+  NFLOW_HIP(hipStreamSynchronize(stream));
+  return 0;
+}
+
+int Batch::node_types(int idx, int* type, int* dirstate) {
+  if (idx < 0 || idx >= (int)lats.size()) return fail(NFLOW_EINVAL, "lattice index out of range");
+  HostLattice& h = lats[idx];
+  const size_t n = h.dev.n;
+  int rc = ensure_scratch(0, 0, 0, 2 * n);
+  if (rc) return rc;
+  NFLOW_HIP(launch_node_type(stream, h.dev, d_types, d_types + n));
+  NFLOW_HIP(hipMemcpyAsync(type, d_types, n * sizeof(int), hipMemcpyDeviceToHost, stream));
+  NFLOW_HIP(hipMemcpyAsync(dirstate, d_types + n, n * sizeof(int), hipMemcpyDeviceToHost, stream));
+  NFLOW_HIP(hipStreamSynchronize(stream));
+  return 0;
+}
+
+// Times every candidate launch shape on this batch's own lattices and keeps
+// the fastest.  The trial steps advance the state, so every lattice is
+// re-initialised afterwards (call before importing a state).
+int Batch::autotune(int trial_steps, nflow_tuning* out) {
+  if (trial_steps < 2) trial_steps = 2;
+  const nflow_tuning base = tuning;
+  nflow_tuning best = base;
+  best.ms_per_step = 1e30;
+  const int blocks[] = {64, 128, 256, 512};
+  const int pers[] = {0, 2, 4, 8};
+  for (int bs : blocks)
+    for (int xo = 0; xo < 2; ++xo)
+      for (int pe : pers) {
+        nflow_tuning t = base;
+        t.block = bs;
+        t.xcd_order = xo;
+        t.persistent = pe;
+        t.graph_steps = 0;
+        int rc = set_tuning(t);
+        if (rc) return rc;
+        rc = init_all();
+        if (rc) return rc;
+        rc = launch_steps(2);
+        if (rc) return rc;
+        rc = run(trial_steps);
+        if (rc) return rc;
+        double ms = 0.0;
+        rc = timing(&ms, nullptr);
+        if (rc) return rc;
+        if (ms < best.ms_per_step) {
+          best = t;
+          best.ms_per_step = ms;
+        }
+      }
+  best.graph_steps = base.graph_steps;
+  int rc = set_tuning(best);
+  if (rc) return rc;
+  tuning.ms_per_step = best.ms_per_step;
+  rc = init_all();
+  if (rc) return rc;
+  NFLOW_HIP(hipStreamSynchronize(stream));
+  if (out) *out = tuning;
+  return 0;
+}
+
+}  // namespace nflow
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace nflow;
+
+struct nflow_solver {
+  Batch b;
+};
+struct nflow_batch {
+  Batch b;
+};
+
+extern "C" const char* nflow_last_error(void) { return last_error_slot().c_str(); }
+
+// Tiles must cover [0,nx) x [0,ny) x [0,nz) exactly once; empty tiles are
+// skipped as slice_chunk's callers do (runSimulation.cpp:38).
+static int tiles_from_slice(const nflow_input& in, int ntile, const int slice[][3][2], std::vector<Tile>& tiles) {
+  tiles.clear();
+  if (ntile <= 0 || !slice) {
+    tiles.push_back(Tile{0, in.nx, 0, in.ny, 0, in.nz});
+    return 0;
+  }
+  long long vol = 0;
+  for (int t = 0; t < ntile; ++t) {
+    Tile tl{slice[t][0][0], slice[t][0][1], slice[t][1][0], slice[t][1][1], slice[t][2][0], slice[t][2][1]};
+    if (tl.x1 <= tl.x0 || tl.y1 <= tl.y0 || tl.z1 <= tl.z0) continue;
+    if (tl.x0 < 0 || tl.y0 < 0 || tl.z0 < 0 || tl.x1 > in.nx || tl.y1 > in.ny || tl.z1 > in.nz)
+      return fail(NFLOW_EINVAL, "tile %d lies outside the %dx%dx%d box", t, in.nx, in.ny, in.nz);
+    vol += (long long)(tl.x1 - tl.x0) * (tl.y1 - tl.y0) * (tl.z1 - tl.z0);
+    tiles.push_back(tl);
+  }
+  const long long n = (long long)in.nx * in.ny * in.nz;
+  if (vol != n) return fail(NFLOW_EINVAL, "tiles cover %lld nodes, the box has %lld", vol, n);
+  // equal volume + no overlap => exact partition
+  std::vector<uint8_t> seen((size_t)n, 0);
+  for (const Tile& tl : tiles)
+    for (int y = tl.y0; y < tl.y1; ++y)
+      for (int z = tl.z0; z < tl.z1; ++z)
+        for (int x = tl.x0; x < tl.x1; ++x) {
+          uint8_t& s = seen[(size_t)x + (size_t)in.nx * ((size_t)z + (size_t)in.nz * y)];
+          if (s) return fail(NFLOW_EINVAL, "tiles overlap at node (%d, %d, %d)", x, y, z);
+          s = 1;
+        }
+  return 0;
+}
+
+extern "C" int nflow_create(const nflow_input* in, int ntile, const int slice[][3][2], nflow_solver** out) {
+  if (!in || !out) return fail(NFLOW_EINVAL, "nflow_create: null argument");
+  *out = nullptr;
+  int rc = validate_input(*in, in->storage);
+  if (rc) return rc;
+  std::vector<Tile> tiles;
+  rc = tiles_from_slice(*in, ntile, slice, tiles);
+  if (rc) return rc;
+  nflow_solver* s = new nflow_solver();
+  s->b.storage = in->storage;
+  rc = s->b.open();
+  if (!rc) rc = s->b.add_lattice(*in, tiles, nullptr);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return 0;
+}
+
+extern "C" int nflow_initialize(nflow_solver* s) {
+  if (!s) return fail(NFLOW_EINVAL, "null solver");
+  int rc = s->b.init_all();
+  if (rc) return rc;
+  return s->b.synchronize();
+}
+
+extern "C" int nflow_run(nflow_solver* s, int nsteps) {
+  if (!s) return fail(NFLOW_EINVAL, "null solver");
+  return s->b.run(nsteps);
+}
+
+extern "C" int nflow_synchronize(nflow_solver* s) {
+  if (!s) return fail(NFLOW_EINVAL, "null solver");
+  return s->b.synchronize();
+}
+
+extern "C" long long nflow_steps_done(const nflow_solver* s) { return s ? s->b.lats[0].steps_done : -1; }
+
+extern "C" int nflow_read_arF(nflow_solver* s, float* dst) {
+  if (!s || !dst) return fail(NFLOW_EINVAL, "nflow_read_arF: null argument");
+  return s->b.export_one(0, dst, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+extern "C" int nflow_write_arF(nflow_solver* s, const float* src) {
+  if (!s || !src) return fail(NFLOW_EINVAL, "nflow_write_arF: null argument");
+  return s->b.import_one(0, src);
+}
+
+extern "C" int nflow_read_macro(nflow_solver* s, float* rho, float* ux, float* uy, float* uz) {
+  if (!s || !rho || !ux || !uy || !uz) return fail(NFLOW_EINVAL, "nflow_read_macro: null argument");
+  return s->b.export_one(0, nullptr, rho, ux, uy, uz, nullptr);
+}
+
+extern "C" int nflow_read_node_type(nflow_solver* s, int* node_type, int* direction_state) {
+  if (!s || !node_type || !direction_state) return fail(NFLOW_EINVAL, "nflow_read_node_type: null argument");
+  return s->b.node_types(0, node_type, direction_state);
+}
+
+extern "C" int nflow_diagnostics(nflow_solver* s, double* mass, double mom[3], double* drho_l1) {
+  if (!s || !mass || !mom || !drho_l1) return fail(NFLOW_EINVAL, "nflow_diagnostics: null argument");
+  double d[5];
+  int rc = s->b.export_one(0, nullptr, nullptr, nullptr, nullptr, nullptr, d);
+  if (rc) return rc;
+  *mass = d[0];
+  mom[0] = d[1];
+  mom[1] = d[2];
+  mom[2] = d[3];
+  *drho_l1 = d[4];
+  return 0;
+}
+
+extern "C" int nflow_timing(nflow_solver* s, double* ms_per_step, int* steps_timed) {
+  if (!s) return fail(NFLOW_EINVAL, "null solver");
+  return s->b.timing(ms_per_step, steps_timed);
+}
+
+extern "C" void nflow_destroy(nflow_solver* s) { delete s; }
+
+// ---- batch -----------------------------------------------------------------
+extern "C" int nflow_batch_create(int count, const nflow_input* inputs, nflow_batch** out) {
+  if (!inputs || !out || count < 1) return fail(NFLOW_EINVAL, "nflow_batch_create: bad argument");
+  *out = nullptr;
+  nflow_batch* b = new nflow_batch();
+  b->b.storage = inputs[0].storage;
+  int rc = b->b.open();
+  for (int i = 0; i < count && !rc; ++i) rc = b->b.add_lattice(inputs[i], {}, nullptr);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+extern "C" int nflow_batch_initialize(nflow_batch* b) {
+  if (!b) return fail(NFLOW_EINVAL, "null batch");
+  int rc = b->b.init_all();
+  return rc ? rc : b->b.synchronize();
+}
+
+extern "C" int nflow_batch_autotune(nflow_batch* b, int steps_per_trial, nflow_tuning* chosen) {
+  if (!b) return fail(NFLOW_EINVAL, "null batch");
+  return b->b.autotune(steps_per_trial, chosen);
+}
+
+extern "C" int nflow_batch_set_tuning(nflow_batch* b, const nflow_tuning* t) {
+  if (!b || !t) return fail(NFLOW_EINVAL, "null argument");
+  return b->b.set_tuning(*t);
+}
+
+extern "C" int nflow_batch_get_tuning(nflow_batch* b, nflow_tuning* t) {
+  if (!b || !t) return fail(NFLOW_EINVAL, "null argument");
+  *t = b->b.tuning;
+  return 0;
+}
+
+extern "C" int nflow_batch_run(nflow_batch* b, int nsteps) {
+  if (!b) return fail(NFLOW_EINVAL, "null batch");
+  return b->b.run(nsteps);
+}
+
+extern "C" int nflow_batch_synchronize(nflow_batch* b) {
+  if (!b) return fail(NFLOW_EINVAL, "null batch");
+  return b->b.synchronize();
+}
+
+extern "C" int nflow_batch_read_arF(nflow_batch* b, int idx, float* dst) {
+  if (!b || !dst) return fail(NFLOW_EINVAL, "null argument");
+  return b->b.export_one(idx, dst, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+extern "C" int nflow_batch_read_macro(nflow_batch* b, int idx, float* rho, float* ux, float* uy, float* uz) {
+  if (!b || !rho || !ux || !uy || !uz) return fail(NFLOW_EINVAL, "null argument");
+  return b->b.export_one(idx, nullptr, rho, ux, uy, uz, nullptr);
+}
+
+extern "C" int nflow_batch_diagnostics(nflow_batch* b, int idx, double* mass, double mom[3], double* drho_l1) {
+  if (!b || !mass || !mom || !drho_l1) return fail(NFLOW_EINVAL, "null argument");
+  double d[5];
+  int rc = b->b.export_one(idx, nullptr, nullptr, nullptr, nullptr, nullptr, d);
+  if (rc) return rc;
+  *mass = d[0];
+  mom[0] = d[1];
+  mom[1] = d[2];
+  mom[2] = d[3];
+  *drho_l1 = d[4];
+  return 0;
+}
+
+extern "C" int nflow_batch_timing(nflow_batch* b, double* ms_per_step, int* steps_timed) {
+  if (!b) return fail(NFLOW_EINVAL, "null batch");
+  return b->b.timing(ms_per_step, steps_timed);
+}
+
+extern "C" long long nflow_batch_nodes(nflow_batch* b) { return b ? b->b.total_nodes() : -1; }
+extern "C" size_t nflow_batch_device_bytes(nflow_batch* b) { return b ? b->b.device_bytes() : 0; }
+extern "C" void nflow_batch_destroy(nflow_batch* b) { delete b; }
+
+// ---- measurement helpers -----------------------------------------------------
+extern "C" int nflow_copy_bandwidth(size_t bytes, int streams, int reps, double* gbps, double* ms_per_copy) {
+  if (streams < 1 || reps < 1 || bytes < (size_t)streams * 4) return fail(NFLOW_EINVAL, "bad copy parameters");
+  const size_t per = bytes / sizeof(float) / streams;
+  float *src = nullptr, *dst = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  float ms = 0.0f;
+  if (hipMalloc(&src, per * streams * sizeof(float)) != hipSuccess ||
+      hipMalloc(&dst, per * streams * sizeof(float)) != hipSuccess) {
+    rc = fail(NFLOW_ENOMEM, "copy buffers");
+    goto done;
+  }
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(NFLOW_EDEVICE, "stream/event creation");
+    goto done;
+  }
+  if (hipMemsetAsync(src, 0, per * streams * sizeof(float), st) != hipSuccess ||
+      launch_copy_streams(st, src, dst, per, streams) != hipSuccess || hipEventRecord(e0, st) != hipSuccess) {
+    rc = fail(NFLOW_EDEVICE, "copy warm-up");
+    goto done;
+  }
+  for (int r = 0; r < reps; ++r)
+    if (launch_copy_streams(st, src, dst, per, streams) != hipSuccess) {
+      rc = fail(NFLOW_EDEVICE, "copy launch");
+      goto done;
+    }
+  if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+      hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+    rc = fail(NFLOW_EDEVICE, "copy timing");
+    goto done;
+  }
+  if (ms_per_copy) *ms_per_copy = ms / reps;
+  if (gbps) *gbps = 2.0 * (double)per * streams * sizeof(float) / ((double)ms / reps * 1e-3) / 1e9;
+done:
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  return rc;
+}
+
+extern "C" int nflow_device_info(char* name, int name_len, int* cu_count, size_t* hbm_bytes) {
+  int dev = 0;
+  NFLOW_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  NFLOW_HIP(hipGetDeviceProperties(&p, dev));
+  if (name && name_len > 0) {
+    std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (cu_count) *cu_count = p.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
+  return 0;
+}

@@ -1,0 +1,408 @@
+// serve.hip -- continuous-batching request loop.
+//
+// One host thread owns the device work.  Requests wait in a FIFO queue and
+// are admitted when a resident slot is free and their device footprint fits
+// the HBM budget (an arena carved out of one allocation, best-fit with
+// coalescing, so admission never calls hipMalloc).  A request that does not
+// fit lets smaller ones behind it be admitted (backfill) until it has waited
+// `max_head_wait_s`; after that nothing overtakes it.  Every tick advances
+// all resident lattices together by k steps, k being the smallest remaining
+// step count (capped by steps_per_tick), so lattices leave exactly at their
+// step count; finished lattices are exported (FP64 diagnostics, optionally
+// rho/u) and their memory returns to the arena before the next admission.
+// A lattice's arithmetic never depends on what else is resident, so a
+// request gives the same result alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "batch.hpp"
+#include "errors.hpp"
+#include "nflow.h"
+
+namespace nflow {
+
+class Arena {
+ public:
+  int open(size_t bytes) {
+    size_ = bytes / 256 * 256;
+    if (size_ == 0) return fail(NFLOW_EINVAL, "server budget too small");
+    if (hipMalloc(&base_, size_) != hipSuccess) {
+      (void)hipGetLastError();
+      base_ = nullptr;
+      return fail(NFLOW_ENOMEM, "hipMalloc of the %zu-byte serving arena failed", size_);
+    }
+    free_[0] = size_;
+    return 0;
+  }
+  ~Arena() {
+    if (base_) (void)hipFree(base_);
+  }
+  size_t size() const { return size_; }
+  // best fit; returns nullptr when no free range is large enough
+  void* alloc(size_t bytes) {
+    bytes = (bytes + 255) / 256 * 256;
+    auto best = free_.end();
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= bytes && (best == free_.end() || it->second < best->second)) best = it;
+    if (best == free_.end()) return nullptr;
+    const size_t off = best->first, len = best->second;
+    free_.erase(best);
+    if (len > bytes) free_[off + bytes] = len - bytes;
+    used_ += bytes;
+    return static_cast<char*>(base_) + off;
+  }
+  void release(void* p, size_t bytes) {
+    bytes = (bytes + 255) / 256 * 256;
+    size_t off = static_cast<char*>(p) - static_cast<char*>(base_);
+    used_ -= bytes;
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && off + bytes == next->first) {
+      bytes += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        prev->second += bytes;
+        return;
+      }
+    }
+    free_[off] = bytes;
+  }
+  size_t used() const { return used_; }
+
+ private:
+  void* base_ = nullptr;
+  size_t size_ = 0, used_ = 0;
+  std::map<size_t, size_t> free_;  // offset -> length
+};
+
+struct Pending {
+  nflow_request req;
+  double submit_s;
+};
+
+struct Resident {
+  nflow_request req;
+  double submit_s, admit_s;
+  long long target;
+};
+
+class Server {
+ public:
+  Batch batch;
+  Arena arena;
+  int device = 0;
+  int max_resident = 64;
+  int steps_per_tick = 100;
+  double max_head_wait_s = 2.0;
+  std::chrono::steady_clock::time_point t0;
+
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Pending> queue;
+  std::deque<nflow_result> results;
+  std::map<unsigned long long, std::vector<float>> fields;
+  std::vector<Resident> resident;  // parallel to batch.lats
+  nflow_server_stats st{};
+  bool stop = false;
+  int fatal = 0;
+  std::string fatal_msg;
+  std::thread worker;
+
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+
+  void loop();
+  void admit(double t);
+  int tick();
+  void finish(size_t i, double t);
+};
+
+void Server::admit(double t) {
+  std::lock_guard<std::mutex> lk(mu);
+  bool head_blocked = false;
+  for (auto it = queue.begin(); it != queue.end();) {
+    if ((int)resident.size() >= max_resident) break;
+    const size_t bytes = lattice_bytes(it->req.input, batch.storage);
+    if (bytes > arena.size()) {  // can never fit
+      nflow_result r{};
+      r.tag = it->req.tag;
+      r.status = NFLOW_EBUDGET;
+      r.nx = it->req.input.nx;
+      r.ny = it->req.input.ny;
+      r.nz = it->req.input.nz;
+      r.submit_s = it->submit_s;
+      r.admit_s = r.done_s = t;
+      results.push_back(r);
+      st.rejected++;
+      it = queue.erase(it);
+      cv_done.notify_all();
+      continue;
+    }
+    const bool is_head = it == queue.begin();
+    if (head_blocked && t - queue.front().submit_s > max_head_wait_s) break;  // no overtaking an old head
+    void* mem = arena.alloc(bytes);
+    if (!mem) {
+      if (is_head) head_blocked = true;
+      ++it;
+      continue;
+    }
+    const int idx = (int)batch.lats.size();
+    int rc = batch.add_lattice(it->req.input, {}, mem);
+    if (!rc) rc = batch.init_lattice(idx, nullptr);
+    if (rc) {
+      if ((int)batch.lats.size() > idx) batch.lats.pop_back();
+      arena.release(mem, bytes);
+      nflow_result r{};
+      r.tag = it->req.tag;
+      r.status = rc;
+      r.submit_s = it->submit_s;
+      r.admit_s = r.done_s = t;
+      results.push_back(r);
+      st.rejected++;
+      it = queue.erase(it);
+      cv_done.notify_all();
+      continue;
+    }
+    resident.push_back(Resident{it->req, it->submit_s, t, (long long)it->req.nsteps});
+    st.admitted++;
+    st.peak_resident = std::max<int>(st.peak_resident, (int)resident.size());
+    st.peak_bytes = std::max(st.peak_bytes, arena.used());
+    it = queue.erase(it);
+  }
+}
+
+void Server::finish(size_t i, double t) {
+  HostLattice& h = batch.lats[i];
+  Resident& r = resident[i];
+  nflow_result res{};
+  res.tag = r.req.tag;
+  res.nx = h.in.nx;
+  res.ny = h.in.ny;
+  res.nz = h.in.nz;
+  res.steps = h.steps_done;
+  res.submit_s = r.submit_s;
+  res.admit_s = r.admit_s;
+  double d[5] = {0, 0, 0, 0, 0};
+  std::vector<float> f;
+  int rc;
+  if (r.req.want_fields) {
+    const size_t n = h.dev.n;
+    f.resize(4 * n);
+    rc = batch.export_one((int)i, nullptr, f.data(), f.data() + n, f.data() + 2 * n, f.data() + 3 * n, d);
+  } else {
+    rc = batch.export_one((int)i, nullptr, nullptr, nullptr, nullptr, nullptr, d);
+  }
+  res.status = rc;
+  res.mass = d[0];
+  res.mom[0] = d[1];
+  res.mom[1] = d[2];
+  res.mom[2] = d[3];
+  res.drho_l1 = d[4];
+  res.done_s = now();
+  (void)t;
+  const long long upd = (long long)h.dev.n * h.steps_done;
+  const bool keep_fields = r.req.want_fields && rc == 0;
+  void* mem = h.mem;
+  const size_t bytes = h.bytes;
+  batch.remove_lattice((int)i);
+  arena.release(mem, bytes);
+  resident.erase(resident.begin() + i);  // r and h are invalid from here on
+  std::lock_guard<std::mutex> lk(mu);
+  if (keep_fields) fields[res.tag] = std::move(f);
+  results.push_back(res);
+  st.completed++;
+  st.node_updates += upd;
+  cv_done.notify_all();
+}
+
+int Server::tick() {
+  long long k = steps_per_tick;
+  for (size_t i = 0; i < resident.size(); ++i)
+    k = std::min(k, resident[i].target - batch.lats[i].steps_done);
+  if (k > 0) {
+    int rc = batch.launch_steps((int)k);
+    if (rc) return rc;
+    st.launches += k;
+  }
+  NFLOW_HIP(hipStreamSynchronize(batch.stream));
+  return 0;
+}
+
+void Server::loop() {
+  (void)hipSetDevice(device);
+  double busy_from = -1.0;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      if (stop) break;  // destroy abandons queued and resident work
+      if (resident.empty() && queue.empty()) {
+        if (busy_from >= 0.0) {
+          st.busy_s += now() - busy_from;
+          busy_from = -1.0;
+        }
+        cv_work.wait_for(lk, std::chrono::milliseconds(50));
+        continue;
+      }
+    }
+    const double t = now();
+    admit(t);
+    if (resident.empty()) {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_work.wait_for(lk, std::chrono::milliseconds(5));
+      continue;
+    }
+    if (busy_from < 0.0) busy_from = t;
+    int rc = tick();
+    if (rc) {
+      std::lock_guard<std::mutex> lk(mu);
+      fatal = rc;
+      fatal_msg = last_error_slot();
+      // fail everything still held
+      for (auto& r : resident) {
+        nflow_result res{};
+        res.tag = r.req.tag;
+        res.status = rc;
+        res.submit_s = r.submit_s;
+        res.admit_s = r.admit_s;
+        res.done_s = now();
+        results.push_back(res);
+      }
+      for (auto& p : queue) {
+        nflow_result res{};
+        res.tag = p.req.tag;
+        res.status = rc;
+        res.submit_s = p.submit_s;
+        results.push_back(res);
+      }
+      queue.clear();
+      resident.clear();
+      cv_done.notify_all();
+      break;
+    }
+    const double t2 = now();
+    for (size_t i = resident.size(); i-- > 0;)
+      if (batch.lats[i].steps_done >= resident[i].target) finish(i, t2);
+  }
+  if (busy_from >= 0.0) st.busy_s += now() - busy_from;
+}
+
+}  // namespace nflow
+
+using namespace nflow;
+
+struct nflow_server {
+  Server s;
+};
+
+extern "C" int nflow_server_create(size_t hbm_budget_bytes, int storage, int max_resident, int steps_per_tick,
+                                   nflow_server** out) {
+  if (!out || max_resident < 1 || steps_per_tick < 1) return fail(NFLOW_EINVAL, "nflow_server_create: bad argument");
+  if (storage < NFLOW_STORAGE_F32 || storage > NFLOW_STORAGE_BF16) return fail(NFLOW_EINVAL, "bad storage");
+  *out = nullptr;
+  nflow_server* sv = new nflow_server();
+  Server& s = sv->s;
+  s.batch.storage = storage;
+  s.max_resident = max_resident;
+  s.steps_per_tick = steps_per_tick;
+  nflow_tuning t = s.batch.tuning;
+  t.graph_steps = 0;  // the resident set changes between ticks
+  s.batch.tuning = t;
+  int rc = hipGetDevice(&s.device) == hipSuccess ? 0 : fail(NFLOW_EDEVICE, "hipGetDevice");
+  if (!rc) rc = s.batch.open();
+  if (!rc) rc = s.arena.open(hbm_budget_bytes);
+  if (rc) {
+    delete sv;
+    return rc;
+  }
+  s.t0 = std::chrono::steady_clock::now();
+  s.worker = std::thread([&s] { s.loop(); });
+  *out = sv;
+  return 0;
+}
+
+extern "C" int nflow_server_submit(nflow_server* sv, const nflow_request* r) {
+  if (!sv || !r) return fail(NFLOW_EINVAL, "nflow_server_submit: null argument");
+  if (r->nsteps < 0) return fail(NFLOW_EINVAL, "nsteps must be >= 0");
+  Server& s = sv->s;
+  int rc = validate_input(r->input, s.batch.storage);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(s.mu);
+  if (s.fatal) return fail(s.fatal, "server stopped: %s", s.fatal_msg.c_str());
+  if (s.stop) return fail(NFLOW_ESTATE, "server is shutting down");
+  Pending p{*r, s.now()};
+  p.req.input.storage = s.batch.storage;
+  s.queue.push_back(p);
+  s.st.submitted++;
+  s.cv_work.notify_all();
+  return 0;
+}
+
+extern "C" int nflow_server_poll(nflow_server* sv, nflow_result* out, int max_results, int wait_ms) {
+  if (!sv || (!out && max_results > 0)) return fail(NFLOW_EINVAL, "nflow_server_poll: null argument");
+  Server& s = sv->s;
+  std::unique_lock<std::mutex> lk(s.mu);
+  if (s.results.empty() && wait_ms > 0)
+    s.cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return !s.results.empty(); });
+  int n = 0;
+  while (n < max_results && !s.results.empty()) {
+    out[n++] = s.results.front();
+    s.results.pop_front();
+  }
+  return n;
+}
+
+extern "C" int nflow_server_fetch_macro(nflow_server* sv, unsigned long long tag, float* rho, float* ux, float* uy,
+                                        float* uz) {
+  if (!sv || !rho || !ux || !uy || !uz) return fail(NFLOW_EINVAL, "nflow_server_fetch_macro: null argument");
+  Server& s = sv->s;
+  std::lock_guard<std::mutex> lk(s.mu);
+  auto it = s.fields.find(tag);
+  if (it == s.fields.end()) return fail(NFLOW_ENOTFOUND, "no fields kept for tag %llu", tag);
+  const size_t n = it->second.size() / 4;
+  std::memcpy(rho, it->second.data(), n * sizeof(float));
+  std::memcpy(ux, it->second.data() + n, n * sizeof(float));
+  std::memcpy(uy, it->second.data() + 2 * n, n * sizeof(float));
+  std::memcpy(uz, it->second.data() + 3 * n, n * sizeof(float));
+  s.fields.erase(it);
+  return 0;
+}
+
+extern "C" int nflow_server_drain(nflow_server* sv) {
+  if (!sv) return fail(NFLOW_EINVAL, "null server");
+  Server& s = sv->s;
+  std::unique_lock<std::mutex> lk(s.mu);
+  for (;;) {
+    if (s.fatal) return fail(s.fatal, "server stopped: %s", s.fatal_msg.c_str());
+    if (s.queue.empty() && s.st.completed + s.st.rejected >= s.st.submitted) return 0;
+    s.cv_done.wait_for(lk, std::chrono::milliseconds(20));
+  }
+}
+
+extern "C" int nflow_server_stats_get(nflow_server* sv, nflow_server_stats* out) {
+  if (!sv || !out) return fail(NFLOW_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(sv->s.mu);
+  *out = sv->s.st;
+  return 0;
+}
+
+extern "C" void nflow_server_destroy(nflow_server* sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> lk(sv->s.mu);
+    sv->s.stop = true;
+    sv->s.cv_work.notify_all();
+  }
+  if (sv->s.worker.joinable()) sv->s.worker.join();
+  delete sv;
+}
