@@ -77,7 +77,7 @@ def cpu_baseline(w, budget_s=15.0):
     t = time.perf_counter()
     L.run(1)
     t1 = time.perf_counter() - t
-    steps = max(1, min(20, int(budget_s / max(t1, 1e-3))))
+    steps = max(1, min(60, int(budget_s / max(t1, 1e-3))))
     t = time.perf_counter()
     L.run(steps)
     dt = time.perf_counter() - t
@@ -88,20 +88,174 @@ def cpu_baseline(w, budget_s=15.0):
                       f"({dt:.1f} s)"}
 
 
+def solo_run(nflow, inp, steps):
+    """One request solved alone: create, initialise, run, diagnostics, destroy."""
+    t = time.perf_counter()
+    s = nflow.Solver(inp)
+    s.initialize()
+    s.run(steps)
+    m, _, _ = s.diagnostics()
+    s.close()
+    return time.perf_counter() - t, m
+
+
+def bench_batch(args):
+    """64 seeded cavities (edges 32..160, Re 100..5000, SRT or MRT-LES), FP16
+    storage, one launch per step over all of them, graph-captured steps."""
+    import nflow
+    import workloads
+
+    lat = workloads.batched_cavities(seed=args.seed, count=64, storage=nflow.F16)
+    inputs = [d["input"] for d in lat]
+    b = nflow.Batch(inputs)
+    tuning = b.autotune(steps_per_trial=10)
+    b.set_tuning(tuning["block"], tuning["xcd_order"], tuning["persistent"], args.graph_steps)
+    b.initialize()
+    b.run(args.warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(args.steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms, _ = b.timing()
+    nodes = b.nodes()
+    diverged = [i for i in range(len(inputs)) if not (abs(b.diagnostics(i)[0]) < float("inf"))]
+    b.close()
+    # the same lattices one at a time with the single-lattice solver
+    solo = {}
+    for storage, name in ((nflow.F16, "f16"), (nflow.F32, "f32")):
+        tot = 0.0
+        for d in lat:
+            inp = workloads.cavity_input(d["edge"], d["re"], d["mrt"], storage)
+            dt, _ = solo_run(nflow, inp, args.steps + args.warmup)
+            tot += dt
+        solo[name] = nodes * (args.steps + args.warmup) / tot / 1e6
+    mlups = nodes * args.steps / wall / 1e6
+    print(json.dumps({
+        "metric": "batched D3Q19 aggregate MLUPS (64 cavities, one launch per step)", "value": round(mlups, 1),
+        "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
+        "dtype": "f16-storage/f32-compute", "data": f"synthetic, seed {args.seed}",
+        "config": {"workload": "64 cavities, edges uniform in {32,48,...,160}, Re log-uniform [100,5000], "
+                               "nu = U L/Re, U = 0.05, SRT or MRT-LES (Cs 0.17) with p = 1/2",
+                   "nodes": nodes, "tuning": tuning, "graph_steps": args.graph_steps,
+                   "srt": sum(1 for d in lat if not d["mrt"])},
+        "device_ms_per_step": round(dev_ms, 4),
+        "roofline": {"bound": "hbm", "achieved": round(76 * nodes / (dev_ms * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(76 * nodes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "solo_f16_mlups": round(solo["f16"], 1), "solo_f32_mlups": round(solo["f32"], 1),
+        "speedup_vs_solo_f32": round(mlups / solo["f32"], 3),
+        "diverged_lattices": diverged,
+    }))
+
+
+def bench_serve(args):
+    """Poisson request stream served by the continuous-batching loop (FP16
+    storage), against the same requests solved one at a time with the plain
+    FP32 solver."""
+    import threading
+
+    import numpy as np
+
+    import nflow
+    import workloads
+
+    reqs = workloads.request_stream(seed=args.seed, count=args.requests, mean_gap_s=0.05,
+                                    step_scale=args.step_scale, storage=nflow.F16)
+    _, _, hbm = nflow.device_info()
+    budget = min(200 << 30, int(hbm * 0.75))
+    srv = nflow.Server(budget, storage=nflow.F16, max_resident=64, steps_per_tick=args.tick)
+    check = sorted(reqs, key=lambda r: r["nodes"])[:4]
+    check_tags = {r["tag"] for r in check}
+    t_start = time.perf_counter()
+
+    def submitter():
+        for r in reqs:
+            delay = r["arrival"] - (time.perf_counter() - t_start)
+            if delay > 0:
+                time.sleep(delay)
+            srv.submit(r["input"], r["steps"], r["tag"], want_fields=r["tag"] in check_tags)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    results = {}
+    while len(results) < len(reqs):
+        for r in srv.poll(64, wait_ms=200):
+            results[r.tag] = dict(status=r.status, submit=r.submit_s, admit=r.admit_s, done=r.done_s)
+    th.join()
+    wall = max(v["done"] for v in results.values()) - min(v["submit"] for v in results.values())
+    st = srv.stats()
+    fields = {t: srv.fetch_macro(t, (r["input"].ny, r["input"].nz, r["input"].nx))
+              for t, r in ((r["tag"], r) for r in check)}
+    srv.close()
+    lat = np.array([v["done"] - v["submit"] for v in results.values()])
+    updates = sum(r["nodes"] * r["steps"] for r in reqs)
+    ok = sum(1 for v in results.values() if v["status"] == 0)
+    # the same requests, one at a time, plain FP32 solver
+    serial = 0.0
+    errs = []
+    for r in reqs:
+        inp = r["input"]
+        plain = nflow.defaults(**{k: getattr(inp, k) for k, _ in inp._fields_ if k != "ref_vel"},
+                               ref_vel=tuple(inp.ref_vel))
+        plain.storage = nflow.F32
+        dt, _ = solo_run(nflow, plain, r["steps"])
+        serial += dt
+        if r["tag"] in check_tags:
+            s = nflow.Solver(plain)
+            s.initialize()
+            s.run(r["steps"])
+            _, ux, uy, uz = s.read_macro()
+            s.close()
+            g = fields[r["tag"]]
+            u32 = np.stack([ux, uy, uz]).astype(np.float64)
+            u16 = np.stack(g[1:]).astype(np.float64)
+            errs.append(float(np.linalg.norm(u16 - u32) / np.linalg.norm(u32)))
+    mlups = updates / wall / 1e6
+    serial_mlups = updates / serial / 1e6
+    print(json.dumps({
+        "metric": "served-batch aggregate MLUPS and p50/p99 latency", "value": round(mlups, 1), "unit": "MLUPS",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f16-storage/f32-compute",
+        "data": f"synthetic Poisson stream, seed {args.seed}",
+        "config": {"workload": "Poisson arrivals mean 50 ms; 1/2 cavity edge U[48,256] Re logU[100,5000] MRT-LES, "
+                               "1/2 channel length U[256,1024] x 128 x 128 MRT-LES; steps U[500,20000] x "
+                               f"{args.step_scale}", "requests": len(reqs), "budget_bytes": budget,
+                   "steps_per_tick": args.tick},
+        "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+        "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+        "completed_ok": ok, "wall_s": round(wall, 3), "node_updates": updates,
+        "serial_plain_f32_mlups": round(serial_mlups, 1), "speedup_vs_serial_plain": round(mlups / serial_mlups, 3),
+        "peak_resident": st["peak_resident"], "peak_bytes": st["peak_bytes"],
+        "rel_l2_u_vs_fp32_solo": [round(e, 6) for e in errs],
+        "latency_includes": "queueing: offered load exceeds capacity",
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="cavity256", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="single", choices=("single", "batch", "serve"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--autotune", action="store_true", help="autotune the launch shape before timing")
+    ap.add_argument("--no-autotune", action="store_true", help="fixed launch shape instead of the autotuner's")
     ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--block", type=int, default=512, help="workgroup size with --no-autotune")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--requests", type=int, default=48, help="serve mode: length of the request stream")
+    ap.add_argument("--step-scale", type=float, default=0.1, help="serve mode: scale of the per-request steps")
+    ap.add_argument("--tick", type=int, default=100, help="serve mode: steps per tick")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab decomposition is "
                                    "a design (DESIGN.md), not code", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.mode == "batch":
+        return bench_batch(args)
+    if args.mode == "serve":
+        return bench_serve(args)
 
     import nflow
 
@@ -111,11 +265,11 @@ def main():
     # the single-lattice solver is a batch of one; use the batch handle so the
     # launch shape can be tuned
     b = nflow.Batch([inp])
-    if args.autotune:
+    if not args.no_autotune:
         tuning = b.autotune(steps_per_trial=10)
         b.set_tuning(tuning["block"], tuning["xcd_order"], tuning["persistent"], args.graph_steps)
     else:
-        b.set_tuning(256, 1, 0, args.graph_steps)
+        b.set_tuning(args.block, 0, 0, args.graph_steps)
     b.initialize()
     b.run(args.warmup)
     b.synchronize()

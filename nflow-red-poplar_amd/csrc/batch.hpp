@@ -40,7 +40,7 @@ class Batch {
   std::vector<HostLattice> lats;
   hipStream_t stream = nullptr;
   int cu_count = 256;
-  nflow_tuning tuning{256, 1, 0, 10, 0.0};
+  nflow_tuning tuning{256, 0, 0, 10, 0.0};
 
   // device tables
   LatticeDev* d_lats = nullptr;

@@ -17,6 +17,12 @@
 
 namespace nflow {
 
+// Extra elements between population planes (0: planes start every
+// round_up(N, 128) elements).
+#ifndef NFLOW_PLANE_PAD
+#define NFLOW_PLANE_PAD 0
+#endif
+
 static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 static size_t elem_size(int storage) { return storage == NFLOW_STORAGE_F32 ? 4 : 2; }
 
@@ -27,7 +33,7 @@ int validate_input(const nflow_input& in, int storage) {
   if (in.casetype == NFLOW_CHANNEL && in.nx < 3) return fail(NFLOW_EINVAL, "channel needs nx >= 3");
   if (storage < NFLOW_STORAGE_F32 || storage > NFLOW_STORAGE_BF16) return fail(NFLOW_EINVAL, "bad storage");
   const unsigned long long n = (unsigned long long)in.nx * in.ny * in.nz;
-  const unsigned long long pstride = round_up(n, 128);
+  const unsigned long long pstride = round_up(n, 128) + NFLOW_PLANE_PAD;
   if (pstride * 19ull >= (1ull << 32)) return fail(NFLOW_EINVAL, "lattice of %llu nodes exceeds the 32-bit index range", n);
   if (!(in.tau > 0.5)) return fail(NFLOW_EINVAL, "tau must exceed 0.5 (got %g)", in.tau);
   if (!(in.ref_den > 0.0)) return fail(NFLOW_EINVAL, "ref_den must be positive");
@@ -42,7 +48,7 @@ struct Layout {
 static Layout layout_of(const nflow_input& in, int storage) {
   Layout l;
   const size_t n = (size_t)in.nx * in.ny * in.nz;
-  l.pstride = round_up(n, 128);
+  l.pstride = round_up(n, 128) + NFLOW_PLANE_PAD;
   l.pop = round_up(19 * l.pstride * elem_size(storage), 256);
   l.side = round_up((size_t)SIDE_PLANES * in.ny * in.nz * sizeof(float), 256);
   l.uin = round_up((size_t)in.ny * in.nz * sizeof(float), 256);
@@ -714,8 +720,8 @@ extern "C" void nflow_batch_destroy(nflow_batch* b) { delete b; }
 
 // ---- measurement helpers -----------------------------------------------------
 extern "C" int nflow_copy_bandwidth(size_t bytes, int streams, int reps, double* gbps, double* ms_per_copy) {
-  if (streams < 1 || reps < 1 || bytes < (size_t)streams * 4) return fail(NFLOW_EINVAL, "bad copy parameters");
-  const size_t per = bytes / sizeof(float) / streams;
+  if (streams < 1 || reps < 1 || bytes < (size_t)streams * 16) return fail(NFLOW_EINVAL, "bad copy parameters");
+  const size_t per = bytes / sizeof(float) / streams / 4 * 4;
   float *src = nullptr, *dst = nullptr;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;

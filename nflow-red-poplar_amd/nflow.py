@@ -299,7 +299,7 @@ class Batch:
         _check(lib().nflow_batch_autotune(self.h, steps_per_trial, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, block=256, xcd_order=1, persistent=0, graph_steps=10):
+    def set_tuning(self, block=256, xcd_order=0, persistent=0, graph_steps=10):
         t = Tuning(block, xcd_order, persistent, graph_steps, 0.0)
         _check(lib().nflow_batch_set_tuning(self.h, ctypes.byref(t)))
 

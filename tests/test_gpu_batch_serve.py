@@ -1,0 +1,146 @@
+"""Batched and served execution: every lattice gives the same result as its
+solo run (bitwise for the same storage), whatever else shares the launches;
+a small lattice of each mix is also checked against the FP64 oracle."""
+import time
+
+import numpy as np
+import pytest
+
+import nflow
+import workloads
+from helpers import TOL_U_F16, cfg, fp32_parity, macro_errors, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def solo_arF(inp, steps):
+    s = nflow.Solver(inp)
+    s.initialize()
+    s.run(steps)
+    a = s.read_arF()
+    s.close()
+    return a
+
+
+def mixed_inputs(storage=nflow.F32):
+    return [
+        nflow.defaults(**cfg(nx=16, ny=12, nz=10, enable_mrt=0, tau=0.6), storage=storage),
+        nflow.defaults(**cfg(nx=33, ny=17, nz=9, enable_les=1, smagorinsky=0.17, tau=0.52), storage=storage),
+        nflow.defaults(**cfg(nx=40, ny=12, nz=10, casetype=0, enable_les=1, smagorinsky=0.17, tau=0.56,
+                             inlet_peak=0.05, perturb=0.01, seed=99), storage=storage),
+        nflow.defaults(**cfg(nx=7, ny=5, nz=3, enable_mrt=1, tau=0.8), storage=storage),
+        nflow.defaults(**cfg(nx=48, ny=48, nz=48, enable_les=1, smagorinsky=0.17, tau=0.6, init_lid_velocity=1),
+                       storage=storage),
+    ]
+
+
+@pytest.mark.parametrize("storage", [nflow.F32, nflow.F16])
+@pytest.mark.parametrize("tuning", [(256, 0, 0, 0), (64, 1, 0, 4), (512, 0, 4, 10), (128, 1, 2, 2)])
+def test_batch_equals_solo_runs(storage, tuning):
+    inputs = mixed_inputs(storage)
+    steps = 37
+    b = nflow.Batch(inputs)
+    b.set_tuning(*tuning)
+    b.initialize()
+    b.run(steps)
+    for i, inp in enumerate(inputs):
+        assert np.array_equal(b.read_arF(i), solo_arF(inp, steps)), (i, tuning)
+
+
+def test_autotuned_batch_equals_solo_runs():
+    inputs = mixed_inputs()
+    b = nflow.Batch(inputs)
+    t = b.autotune(steps_per_trial=4)
+    assert t["block"] in (64, 128, 256, 512) and t["ms_per_step"] > 0
+    b.run(25)  # autotune leaves every lattice re-initialised
+    for i, inp in enumerate(inputs):
+        assert np.array_equal(b.read_arF(i), solo_arF(inp, 25)), i
+
+
+def test_batched_cavities_against_oracle():
+    """Config-4 style mix (seeded sizes/Re, SRT or MRT-LES), FP16 storage; the
+    smallest lattice is compared with the FP64 oracle, the rest with solo runs."""
+    lat = workloads.batched_cavities(seed=3, count=6, storage=nflow.F16)
+    for d in lat:  # keep the test small: shrink edges, keep Re/model draws
+        e = 16 + d["edge"] // 8
+        nu = 0.05 * e / d["re"]
+        d["input"].nx = d["input"].ny = d["input"].nz = d["input"].h = e
+        d["input"].tau = 3 * nu + 0.5
+    inputs = [d["input"] for d in lat]
+    b = nflow.Batch(inputs)
+    b.autotune(steps_per_trial=4)
+    steps = 200
+    b.run(steps)
+    k = int(np.argmax([i.tau for i in inputs]))  # the most stable lattice (SRT near tau = 0.5 may diverge)
+    inp = inputs[k]
+    c = dict(nx=inp.nx, ny=inp.ny, nz=inp.nz, tau=inp.tau, enable_mrt=inp.enable_mrt, enable_les=inp.enable_les,
+             smagorinsky=inp.smagorinsky, ref_vel=(0.05, 0.0, 0.0), init_lid_velocity=0, casetype=1)
+    L = run_oracle(c, steps)
+    eu, _ = macro_errors(b.read_macro(k), L.read_macro())
+    print(f"lattice {k} ({inp.nx}^3, mrt={inp.enable_mrt}, tau={inp.tau:.4f}): rel-L2 u {eu:.2e}")
+    assert eu < TOL_U_F16
+    for i, inp in enumerate(inputs):
+        s = nflow.Solver(inp)
+        s.initialize()
+        s.run(steps)
+        assert np.array_equal(b.read_macro(i)[1], s.read_macro()[1], equal_nan=True), i
+
+
+def test_server_results_equal_solo_results():
+    inputs = mixed_inputs(nflow.F16)
+    steps = [30, 5, 41, 1, 17]
+    srv = nflow.Server(budget_bytes=512 << 20, storage=nflow.F16, max_resident=3, steps_per_tick=8)
+    for i, (inp, n) in enumerate(zip(inputs, steps)):
+        srv.submit(inp, n, tag=100 + i, want_fields=True)
+        time.sleep(0.002 * i)
+    srv.drain()
+    res = {r.tag: r for r in srv.poll(16)}
+    assert sorted(res) == [100 + i for i in range(5)]
+    for i, (inp, n) in enumerate(zip(inputs, steps)):
+        r = res[100 + i]
+        assert r.status == 0 and r.steps == n and r.done_s >= r.admit_s >= r.submit_s
+        shape = (inp.ny, inp.nz, inp.nx)
+        got = srv.fetch_macro(100 + i, shape)
+        s = nflow.Solver(inp)
+        s.initialize()
+        s.run(n)
+        want = s.read_macro()
+        for a, w in zip(got, want):
+            assert np.array_equal(a, w), i
+        m, mom, d = s.diagnostics()
+        assert r.mass == m and list(r.mom) == list(mom)
+    st = srv.stats()
+    assert st["completed"] == 5 and st["peak_resident"] <= 3
+    srv.close()
+
+
+def test_server_budget_admission():
+    small = nflow.defaults(**cfg(nx=16, ny=16, nz=16))
+    big = nflow.defaults(**cfg(nx=200, ny=200, nz=200))  # ~2.4 GB at FP16, over budget
+    srv = nflow.Server(budget_bytes=64 << 20, storage=nflow.F16, max_resident=8, steps_per_tick=4)
+    srv.submit(small, 10, tag=1)
+    srv.submit(big, 10, tag=2)
+    srv.submit(small, 3, tag=3)
+    srv.drain()
+    res = {r.tag: r for r in srv.poll(8)}
+    assert res[1].status == 0 and res[3].status == 0
+    assert res[2].status == -5  # NFLOW_EBUDGET
+    st = srv.stats()
+    assert st["rejected"] == 1 and st["peak_bytes"] <= 64 << 20
+    srv.close()
+
+
+def test_server_backfills_behind_a_blocked_request():
+    """A request that must wait for memory does not stop smaller ones that fit."""
+    a = nflow.defaults(**cfg(nx=64, ny=64, nz=64))
+    srv = nflow.Server(budget_bytes=30 << 20, storage=nflow.F16, max_resident=8, steps_per_tick=2)
+    srv.submit(a, 400, tag=1)   # ~20 MB, admitted first
+    srv.submit(a, 10, tag=2)    # does not fit until 1 leaves
+    small = nflow.defaults(**cfg(nx=12, ny=12, nz=12))
+    srv.submit(small, 10, tag=3)  # fits beside 1
+    srv.drain()
+    res = {r.tag: r for r in srv.poll(8)}
+    assert all(r.status == 0 for r in res.values())
+    assert res[3].admit_s < res[1].done_s  # backfilled while 1 was resident
+    assert res[2].admit_s >= res[1].done_s - 1e-3
+    srv.close()
