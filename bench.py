@@ -232,13 +232,61 @@ def bench_serve(args):
     }))
 
 
+def bench_precision(args):
+    """Storage precision against the FP64 oracle on the channel and cavity
+    configurations: rel-L2 of rho and u per storage format, and MLUPS."""
+    import numpy as np
+
+    import nflow
+
+    os.environ.setdefault("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16)))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    cases = [
+        ("channel 512x128x128 MRT-LES Cs 0.17 tau 0.56, inlet peak 0.05 + 1% seeded", args.channel_steps,
+         dict(nx=512, ny=128, nz=128, casetype=0, enable_mrt=1, enable_les=1, smagorinsky=0.17, tau=0.56,
+              inlet_peak=0.05, perturb=0.01, seed=2024, ref_vel=(0.05, 0.0, 0.0), init_lid_velocity=0)),
+        ("cavity 128^3 MRT-LES Cs 0.17 tau 0.6, lid 0.05", args.cavity_steps,
+         dict(nx=128, ny=128, nz=128, casetype=1, enable_mrt=1, enable_les=1, smagorinsky=0.17, tau=0.6,
+              ref_vel=(0.05, 0.0, 0.0), init_lid_velocity=0)),
+    ]
+    table = []
+    for name, steps, c in cases:
+        L = oracle.Lattice(oracle.default_params(**c))
+        L.initialize()
+        L.run(steps)
+        ro, xo, yo, zo = L.read_macro()
+        del L
+        uo = np.stack([xo, yo, zo])
+        for storage, sname in ((nflow.F32, "f32"), (nflow.F16, "f16-shifted"), (nflow.BF16, "bf16-shifted")):
+            b = nflow.Batch([nflow.defaults(storage=storage, **c)])
+            b.set_tuning(256 if storage else 512, 0, 0, 10)
+            b.initialize()
+            b.run(steps)
+            ms, _ = b.timing()
+            r, ux, uy, uz = b.read_macro(0)
+            b.close()
+            u = np.stack([ux, uy, uz]).astype(np.float64)
+            r = r.astype(np.float64)
+            table.append({"case": name, "steps": steps, "storage": sname,
+                          "rel_l2_u": float(np.linalg.norm(u - uo) / np.linalg.norm(uo)),
+                          "rel_l2_rho": float(np.linalg.norm(r - ro) / np.linalg.norm(ro)),
+                          "rel_l2_rho_minus_1": float(np.linalg.norm(r - ro) / np.linalg.norm(ro - 1.0)),
+                          "mlups": round(c["nx"] * c["ny"] * c["nz"] / (ms * 1e-3) / 1e6, 1)})
+    print(json.dumps({"metric": "rel-L2 error of rho and u against the FP64 oracle per storage format",
+                      "unit": "relative L2", "n_gpus": 1, "data": "synthetic", "results": table}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="cavity256", choices=sorted(WORKLOADS))
-    ap.add_argument("--mode", default="single", choices=("single", "batch", "serve"))
+    ap.add_argument("--mode", default="single", choices=("single", "batch", "serve", "precision"))
+    ap.add_argument("--channel-steps", type=int, default=200, help="precision mode: channel steps")
+    ap.add_argument("--cavity-steps", type=int, default=1000, help="precision mode: cavity steps")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true", help="fixed launch shape instead of the autotuner's")
     ap.add_argument("--graph-steps", type=int, default=10)
@@ -256,6 +304,8 @@ def main():
         return bench_batch(args)
     if args.mode == "serve":
         return bench_serve(args)
+    if args.mode == "precision":
+        return bench_precision(args)
 
     import nflow
 

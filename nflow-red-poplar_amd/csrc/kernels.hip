@@ -11,9 +11,10 @@
 // post-collision populations with unit-stride stores.  The reference runs
 // the same maths as five (Poplar) or five-to-seven (CUDA) passes per step.
 //
-// Populations are SoA planes (x fastest) in FP32, or as FP16 / BF16 of
-// f - w_q with FP32 arithmetic.  Per node and step the kernel reads 19 and
-// writes 19 populations: 152 B at FP32, 76 B at FP16/BF16.
+// Populations are stored in 256-byte runs of consecutive nodes (x fastest)
+// per population (see NFLOW_AOSOA) in FP32, or as FP16 / BF16 of f - w_q
+// with FP32 arithmetic.  Per node and step the kernel reads 19 and writes 19
+// populations: 152 B at FP32, 76 B at FP16/BF16.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -37,24 +38,57 @@
 #define NFLOW_WAVES_PER_EU 0
 #endif
 
+// Population layout.  NFLOW_AOSOA = 1 (default): blocks of B consecutive
+// nodes, each block holding its 19 populations as 19 runs of B elements with
+// B * sizeof(element) = 256 bytes (B = 64 at FP32, 128 at FP16/BF16).  A
+// workgroup's stores then form one contiguous range and its pulls come from
+// a handful of neighbour blocks, so the device sees ~10 concurrent DRAM
+// streams instead of the 38 of plane-per-population SoA.  Measured at 256^3
+// MRT-LES FP32 (512-thread workgroups): SoA 5.75 TB/s, B = 32 5.91,
+// B = 64 6.05, B = 128 6.01, B = 256 5.87.  NFLOW_AOSOA = 0: SoA planes
+// (element q * P + n).
+#ifndef NFLOW_AOSOA
+#define NFLOW_AOSOA 1
+#endif
+
 namespace nflow {
 using namespace d3q19;
 
+// element index of population q of node n for element type T (ps = plane
+// stride, SoA only)
+template <class T>
+__device__ __forceinline__ uint32_t pidx(int q, uint32_t n, uint32_t ps) {
+#if NFLOW_AOSOA
+  constexpr uint32_t B = 256 / sizeof(T);
+  return ((n / B) * Q + (uint32_t)q) * B + (n & (B - 1));
+#else
+  return (uint32_t)q * ps + n;
+#endif
+}
+
+// The population buffers live in device global memory; saying so (address
+// space 1) lets the compiler emit global_load/global_store instead of FLAT
+// accesses, which it must otherwise use for pointers read out of a struct.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+
 template <class T>
 __device__ inline T ld(const T* __restrict__ p, uint32_t i) {
+  const gptr<const T> g = (gptr<const T>)p;
 #if NFLOW_NT_LOAD
-  return __builtin_nontemporal_load(p + i);
+  return __builtin_nontemporal_load(g + i);
 #else
-  return p[i];
+  return g[i];
 #endif
 }
 
 template <class T>
 __device__ inline void st(T* __restrict__ p, uint32_t i, T v) {
+  const gptr<T> g = (gptr<T>)p;
 #if NFLOW_NT_STORE
-  __builtin_nontemporal_store(v, p + i);
+  __builtin_nontemporal_store(v, g + i);
 #else
-  p[i] = v;
+  g[i] = v;
 #endif
 }
 
@@ -139,8 +173,8 @@ __device__ inline int node_type(const LatticeDev& L, int x, int y, int z) {
 // leaves the box): the rule of the node's type applied to f*_opp(q)(x).
 // ---------------------------------------------------------------------------
 template <class St>
-__device__ inline void gather(const LatticeDev& L, const typename St::T* __restrict__ fin,
-                              const float* __restrict__ side, uint32_t n, int x, int y, int z, float f[Q]) {
+__device__ inline void pull(const LatticeDev& L, const typename St::T* __restrict__ fin, uint32_t n, int x, int y,
+                            int z, float f[Q]) {
   const bool xlo = x == 0, xhi = x == L.nx - 1;
   const bool ylo = y == 0, yhi = y == L.ny - 1;
   const bool zlo = z == 0, zhi = z == L.nz - 1;
@@ -150,20 +184,29 @@ __device__ inline void gather(const LatticeDev& L, const typename St::T* __restr
     const bool out = (EX[q] > 0 && xlo) || (EX[q] < 0 && xhi) || (EY[q] > 0 && ylo) || (EY[q] < 0 && yhi) ||
                      (EZ[q] > 0 && zlo) || (EZ[q] < 0 && zhi);
     const int off = EX[q] + EZ[q] * L.nx + EY[q] * L.nxz;
-    const uint32_t idx = out ? (uint32_t)OPP[q] * ps + n : (uint32_t)q * ps + (uint32_t)((int)n - off);
+    const uint32_t idx =
+        out ? pidx<typename St::T>(OPP[q], n, ps) : pidx<typename St::T>(q, (uint32_t)((int)n - off), ps);
     f[q] = St::load(fin, idx) + shift_of<St>(q);
   }
+}
+
+// Wall / inlet / outlet rules on the links pull() filled with f*_opp(q)(x).
+__device__ inline void boundary(const LatticeDev& L, const float* __restrict__ side, int x, int y, int z, float f[Q]) {
+  const bool xlo = x == 0, xhi = x == L.nx - 1;
+  const bool ylo = y == 0, yhi = y == L.ny - 1;
+  const bool zlo = z == 0, zhi = z == L.nz - 1;
   if (xlo | xhi | ylo | yhi | zlo | zhi) {
     const int t = node_type(L, x, y, z);
     if (t == NODE_FIXEDWALL) return;  // plain bounce-back: already in place
+    const gptr<const float> sd = (gptr<const float>)side;
     float a = 0.0f, ubx = 0.0f, uby = 0.0f, ubz = 0.0f, usq = 0.0f;
     if (t == NODE_INLET) {
       const int yz = y * L.nz + z;
-      a = 6.0f * side[SIDE_RHO_IN * L.ny * L.nz + yz] * L.uin[yz];  // rho(x+1) of the previous macroscopic
+      a = 6.0f * sd[SIDE_RHO_IN * L.ny * L.nz + yz] * ((gptr<const float>)L.uin)[yz];  // rho(x+1), previous step
     } else if (t == NODE_OUTLET) {
       const int yz = y * L.nz + z, pl = L.ny * L.nz;
-      const float* u0 = side + SIDE_U_OUT * pl + yz;
-      const float* u1 = side + SIDE_U_OUT1 * pl + yz;
+      const gptr<const float> u0 = sd + SIDE_U_OUT * pl + yz;
+      const gptr<const float> u1 = sd + SIDE_U_OUT1 * pl + yz;
       ubx = u0[0] + 0.5f * (u0[0] - u1[0]);
       uby = u0[pl] + 0.5f * (u0[pl] - u1[pl]);
       ubz = u0[2 * pl] + 0.5f * (u0[2 * pl] - u1[2 * pl]);
@@ -184,6 +227,13 @@ __device__ inline void gather(const LatticeDev& L, const typename St::T* __restr
       }
     }
   }
+}
+
+template <class St>
+__device__ inline void gather(const LatticeDev& L, const typename St::T* __restrict__ fin,
+                              const float* __restrict__ side, uint32_t n, int x, int y, int z, float f[Q]) {
+  pull<St>(L, fin, n, x, y, z, f);
+  boundary(L, side, x, y, z, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -370,16 +420,17 @@ __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], typename
   const float ir = 1.0f / rho;
   if (L.casetype == CASE_CHANNEL) {
     const int pl = L.ny * L.nz, yz = y * L.nz + z;
-    if (x == 1) side_out[SIDE_RHO_IN * pl + yz] = rho;
+    const gptr<float> so = (gptr<float>)side_out;
+    if (x == 1) so[SIDE_RHO_IN * pl + yz] = rho;
     if (x == L.nx - 1) {
-      side_out[(SIDE_U_OUT + 0) * pl + yz] = jx * ir;
-      side_out[(SIDE_U_OUT + 1) * pl + yz] = jy * ir;
-      side_out[(SIDE_U_OUT + 2) * pl + yz] = jz * ir;
+      so[(SIDE_U_OUT + 0) * pl + yz] = jx * ir;
+      so[(SIDE_U_OUT + 1) * pl + yz] = jy * ir;
+      so[(SIDE_U_OUT + 2) * pl + yz] = jz * ir;
     }
     if (x == L.nx - 2) {
-      side_out[(SIDE_U_OUT1 + 0) * pl + yz] = jx * ir;
-      side_out[(SIDE_U_OUT1 + 1) * pl + yz] = jy * ir;
-      side_out[(SIDE_U_OUT1 + 2) * pl + yz] = jz * ir;
+      so[(SIDE_U_OUT1 + 0) * pl + yz] = jx * ir;
+      so[(SIDE_U_OUT1 + 1) * pl + yz] = jy * ir;
+      so[(SIDE_U_OUT1 + 2) * pl + yz] = jz * ir;
     }
   }
   float te = 0.0f;
@@ -391,7 +442,7 @@ __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], typename
   }
   const uint32_t ps = (uint32_t)L.pstride;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) St::store(fout, (uint32_t)q * ps + n, f[q] - shift_of<St>(q));
+  for (int q = 0; q < Q; ++q) St::store(fout, pidx<typename St::T>(q, n, ps), f[q] - shift_of<St>(q));
 }
 
 __device__ inline void coords(const LatticeDev& L, uint32_t n, int& x, int& y, int& z) {
@@ -461,7 +512,7 @@ __global__ void __launch_bounds__(256) k_init(LatticeDev L, int cur, const float
   typename St::T* streamed = static_cast<typename St::T*>(L.buf[cur ^ 1]);
   const uint32_t ps = (uint32_t)L.pstride;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) St::store(streamed, (uint32_t)q * ps + n, f[q] - shift_of<St>(q));
+  for (int q = 0; q < Q; ++q) St::store(streamed, pidx<typename St::T>(q, n, ps), f[q] - shift_of<St>(q));
   relax_and_store<St>(L, f, static_cast<typename St::T*>(L.buf[cur]), L.side[cur], n, x, y, z);
 }
 
@@ -501,7 +552,7 @@ __global__ void __launch_bounds__(EXPORT_BS) k_export(LatticeDev L, int prev, in
     float f[Q];
     if (prev_streamed) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) f[q] = St::load(pb, (uint32_t)q * ps + n) + shift_of<St>(q);
+      for (int q = 0; q < Q; ++q) f[q] = St::load(pb, pidx<typename St::T>(q, n, ps)) + shift_of<St>(q);
     } else {
       gather<St>(L, pb, L.side[prev], n, x, y, z, f);
     }
@@ -526,7 +577,7 @@ __global__ void __launch_bounds__(EXPORT_BS) k_export(LatticeDev L, int prev, in
       if (!prev_streamed) {
         // the collision conserves mass, so sum_q f*_q(prev) is the previous step's density
 #pragma unroll
-        for (int q = 0; q < Q; ++q) ro += (double)(St::load(pb, (uint32_t)q * ps + n) + shift_of<St>(q));
+        for (int q = 0; q < Q; ++q) ro += (double)(St::load(pb, pidx<typename St::T>(q, n, ps)) + shift_of<St>(q));
       }
       dm = rn;
       dx = (double)jx;

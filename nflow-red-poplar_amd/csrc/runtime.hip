@@ -226,7 +226,7 @@ int Batch::upload_tables() {
   if (!tables_dirty) return 0;
   NFLOW_HIP(hipStreamSynchronize(stream));
   destroy_graphs();
-  const int bs = tuning.block;
+  const int bs = tuning.block;  // nodes per work item
   std::vector<WorkItem> items;
   std::vector<LatticeDev> devs(lats.size());
   for (size_t li = 0; li < lats.size(); ++li) {

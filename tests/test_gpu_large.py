@@ -1,0 +1,48 @@
+"""Parity at the benchmark sizes: short runs against the FP64 oracle (with the
+FP32-oracle round-off allowance) and size-independent properties over longer
+runs (mass conservation, mirror symmetry, determinism)."""
+import numpy as np
+import pytest
+
+import nflow
+from helpers import cfg, fp32_parity, run_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def test_cavity_128_mrt_les_tau_051_against_oracle():
+    fp32_parity(cfg(nx=128, ny=128, nz=128, enable_les=1, smagorinsky=0.17, tau=0.51), 20)
+
+
+def test_cavity_256_mrt_les_against_oracle():
+    fp32_parity(cfg(nx=256, ny=256, nz=256, enable_les=1, smagorinsky=0.17, tau=0.6), 3)
+
+
+def test_channel_512x128x128_against_oracle():
+    c = cfg(nx=512, ny=128, nz=128, casetype=0, enable_les=1, smagorinsky=0.17, tau=0.56, inlet_peak=0.05,
+            perturb=0.01, seed=2024)
+    fp32_parity(c, 12)
+
+
+def test_cavity_256_conserves_mass_and_mirror_symmetry():
+    """The cavity conserves mass exactly in exact arithmetic and is symmetric
+    under z -> nz-1-z (u_z -> -u_z); FP32 keeps both to round-off."""
+    n = 256
+    s = run_gpu(cfg(nx=n, ny=n, nz=n, enable_les=1, smagorinsky=0.17, tau=0.6), 1000)
+    m, mom, _ = s.diagnostics()
+    assert abs(m - n ** 3) / n ** 3 < 2e-6
+    r, ux, uy, uz = s.read_macro()
+    assert np.isfinite(ux).all()
+    flip = ux[:, ::-1, :]
+    assert np.linalg.norm(ux - flip) / np.linalg.norm(ux) < 1e-4
+    assert np.linalg.norm(uz + uz[:, ::-1, :]) / np.linalg.norm(ux) < 1e-4
+    assert abs(mom[2]) / abs(mom[0]) < 1e-4
+
+
+@pytest.mark.parametrize("storage", [nflow.F32, nflow.F16])
+def test_cavity_256_is_deterministic(storage):
+    c = cfg(nx=256, ny=256, nz=256, enable_les=1, smagorinsky=0.17, tau=0.6)
+    a = run_gpu(c, 120, storage=storage).read_macro()
+    b = run_gpu(c, 120, storage=storage).read_macro()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
