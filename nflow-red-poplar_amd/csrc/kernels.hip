@@ -55,15 +55,30 @@ namespace nflow {
 using namespace d3q19;
 
 // element index of population q of node n for element type T (ps = plane
-// stride, SoA only)
+// stride, SoA only) = node_part(n) + plane_part(q)
 template <class T>
-__device__ __forceinline__ uint32_t pidx(int q, uint32_t n, uint32_t ps) {
+__device__ __forceinline__ uint32_t node_part(uint32_t n) {
 #if NFLOW_AOSOA
   constexpr uint32_t B = 256 / sizeof(T);
-  return ((n / B) * Q + (uint32_t)q) * B + (n & (B - 1));
+  return (n / B) * (Q * B) + (n & (B - 1));
 #else
-  return (uint32_t)q * ps + n;
+  return n;
 #endif
+}
+
+template <class T>
+__device__ __forceinline__ uint32_t plane_part(uint32_t q, uint32_t ps) {
+#if NFLOW_AOSOA
+  (void)ps;
+  return q * (256 / sizeof(T));
+#else
+  return q * ps;
+#endif
+}
+
+template <class T>
+__device__ __forceinline__ uint32_t pidx(int q, uint32_t n, uint32_t ps) {
+  return node_part<T>(n) + plane_part<T>((uint32_t)q, ps);
 }
 
 // The population buffers live in device global memory; saying so (address
@@ -109,21 +124,13 @@ struct StF16 {
   __device__ static inline void store(T* __restrict__ p, uint32_t i, float v) { st(p, i, (_Float16)v); }
 };
 
+// gfx950 converts float -> bf16 (round to nearest even) in one instruction
+// (v_cvt_pk_bf16_f32); bf16 -> float is a 16-bit shift.
 struct StBF16 {
-  using T = uint16_t;
+  using T = __bf16;
   static constexpr bool shifted = true;
-  __device__ static inline float load(const T* __restrict__ p, uint32_t i) {
-    return __uint_as_float((uint32_t)ld(p, i) << 16);
-  }
-  __device__ static inline void store(T* __restrict__ p, uint32_t i, float v) {
-    uint32_t u = __float_as_uint(v);
-    if ((u & 0x7fffffffu) > 0x7f800000u) {
-      st(p, i, (uint16_t)((u >> 16) | 0x40u));  // quiet NaN
-    } else {
-      u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-      st(p, i, (uint16_t)(u >> 16));
-    }
-  }
+  __device__ static inline float load(const T* __restrict__ p, uint32_t i) { return (float)ld(p, i); }
+  __device__ static inline void store(T* __restrict__ p, uint32_t i, float v) { st(p, i, (__bf16)v); }
 };
 
 template <class St>
@@ -183,6 +190,9 @@ __device__ inline void pull(const LatticeDev& L, const typename St::T* __restric
   for (int q = 0; q < Q; ++q) {
     const bool out = (EX[q] > 0 && xlo) || (EX[q] < 0 && xhi) || (EY[q] > 0 && ylo) || (EY[q] < 0 && yhi) ||
                      (EZ[q] > 0 && zlo) || (EZ[q] < 0 && zhi);
+    // (a select of two full index computations: measured faster than
+    // selecting the source node and the plane separately, which needs more
+    // registers)
     const int off = EX[q] + EZ[q] * L.nx + EY[q] * L.nxz;
     const uint32_t idx =
         out ? pidx<typename St::T>(OPP[q], n, ps) : pidx<typename St::T>(q, (uint32_t)((int)n - off), ps);
