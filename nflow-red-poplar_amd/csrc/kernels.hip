@@ -55,30 +55,15 @@ namespace nflow {
 using namespace d3q19;
 
 // element index of population q of node n for element type T (ps = plane
-// stride, SoA only) = node_part(n) + plane_part(q)
-template <class T>
-__device__ __forceinline__ uint32_t node_part(uint32_t n) {
-#if NFLOW_AOSOA
-  constexpr uint32_t B = 256 / sizeof(T);
-  return (n / B) * (Q * B) + (n & (B - 1));
-#else
-  return n;
-#endif
-}
-
-template <class T>
-__device__ __forceinline__ uint32_t plane_part(uint32_t q, uint32_t ps) {
-#if NFLOW_AOSOA
-  (void)ps;
-  return q * (256 / sizeof(T));
-#else
-  return q * ps;
-#endif
-}
-
+// stride, SoA only)
 template <class T>
 __device__ __forceinline__ uint32_t pidx(int q, uint32_t n, uint32_t ps) {
-  return node_part<T>(n) + plane_part<T>((uint32_t)q, ps);
+#if NFLOW_AOSOA
+  constexpr uint32_t B = 256 / sizeof(T);
+  return ((n / B) * Q + (uint32_t)q) * B + (n & (B - 1));
+#else
+  return (uint32_t)q * ps + n;
+#endif
 }
 
 // The population buffers live in device global memory; saying so (address
