@@ -134,6 +134,24 @@ int nflow_timing(nflow_solver* s, double* ms_per_step, int* steps_timed);
 void nflow_destroy(nflow_solver* s);
 const char* nflow_last_error(void);
 
+/* ---- restart files (Inputval.irestart/irst, struct.h:84; kernel.cu:706) ----
+ * A checkpoint is the exported arF plus the step count: loading it into a
+ * solver of the same extents continues the run exactly where it was saved
+ * (bitwise for FP32 storage). */
+int nflow_save_checkpoint(nflow_solver* s, const char* path);
+int nflow_load_checkpoint(nflow_solver* s, const char* path);
+int nflow_checkpoint_info(const char* path, int* nx, int* ny, int* nz, long long* steps);
+
+/* ---- output files (saveFile, kernel.cu:699-1018; host only) -----------------
+ * VTK unstructured-grid file (binary appended): points, hexahedral cells,
+ * Velocity, Density and, when node_type is given, the six node-flag fields.
+ * Arrays are [ny][nz][nx]. */
+int nflow_write_vtu(const char* path, int nx, int ny, int nz, const float* rho, const float* ux, const float* uy,
+                    const float* uz, const int* node_type);
+/* result.csv: "x,y,z,rho,u,v,w", one line per node. */
+int nflow_write_csv(const char* path, int nx, int ny, int nz, const float* rho, const float* ux, const float* uy,
+                    const float* uz);
+
 /* ---- many lattices (no reference counterpart) ------------------------------
  * Independent lattices of any sizes advanced together: one launch per time
  * step covers every lattice; steps are replayed from a captured HIP graph;

@@ -25,6 +25,9 @@ struct HostLattice {
   long long steps_done = 0;
   int cur = 0;  // buffer holding the newest post-collision state
   bool initialized = false;
+  // buf[cur ^ 1] holds the streamed state written by init/import (no step
+  // launched since), rather than the previous post-collision state
+  bool prev_streamed = true;
 };
 
 int validate_input(const nflow_input& in, int storage);

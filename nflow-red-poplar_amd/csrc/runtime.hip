@@ -292,6 +292,7 @@ int Batch::init_lattice(int idx, const float* d_src) {
   NFLOW_HIP(launch_init(storage, stream, h.dev, gparity, d_src));
   h.cur = gparity;
   h.steps_done = 0;
+  h.prev_streamed = true;
   h.initialized = true;
   return 0;
 }
@@ -351,6 +352,7 @@ int Batch::launch_steps(int nsteps) {
   }
   for (auto& h : lats) {
     h.steps_done += nsteps;
+    if (nsteps > 0) h.prev_streamed = false;
     h.cur = gparity;
   }
   return 0;
@@ -423,7 +425,7 @@ int Batch::export_one(int idx, float* aos, float* rho, float* ux, float* uy, flo
   if (rc) return rc;
   if (aos) NFLOW_HIP(hipMemsetAsync(d_aos, 0, padded_count(h.in) * sizeof(float), stream));
   const int prev = h.cur ^ 1;
-  const int prev_streamed = h.steps_done == 0 ? 1 : 0;
+  const int prev_streamed = h.prev_streamed ? 1 : 0;
   int nblocks = 0;
   float* m = rho ? d_macro : nullptr;
   NFLOW_HIP(launch_export(storage, stream, h.dev, prev, prev_streamed, aos ? d_aos : nullptr, m, m ? m + n : nullptr,
@@ -637,6 +639,92 @@ extern "C" int nflow_timing(nflow_solver* s, double* ms_per_step, int* steps_tim
 }
 
 extern "C" void nflow_destroy(nflow_solver* s) { delete s; }
+
+// ---- checkpoint files --------------------------------------------------------
+// Little-endian: 8-byte magic "NFLWCKP1", int32 header size (64), int32
+// version (1), int32 nx, ny, nz, casetype, int64 steps, int32 storage, zero
+// padding to 64 bytes, then arF as float32 [ny+2][nz+2][nx+2][19].
+namespace {
+constexpr char CKP_MAGIC[8] = {'N', 'F', 'L', 'W', 'C', 'K', 'P', '1'};
+struct CkpHeader {
+  char magic[8];
+  int32_t header_bytes, version, nx, ny, nz, casetype;
+  int64_t steps;
+  int32_t storage;
+  char pad[20];
+};
+static_assert(sizeof(CkpHeader) == 64, "checkpoint header is 64 bytes");
+
+int read_header(FILE* f, const char* path, CkpHeader& h) {
+  if (std::fread(&h, sizeof(h), 1, f) != 1) return fail(NFLOW_EINVAL, "%s: short checkpoint header", path);
+  if (std::memcmp(h.magic, CKP_MAGIC, 8) != 0 || h.header_bytes != 64 || h.version != 1)
+    return fail(NFLOW_EINVAL, "%s is not an nflow checkpoint", path);
+  if (h.nx < 1 || h.ny < 1 || h.nz < 1 || h.steps < 0) return fail(NFLOW_EINVAL, "%s: corrupt header", path);
+  return 0;
+}
+}  // namespace
+
+extern "C" int nflow_checkpoint_info(const char* path, int* nx, int* ny, int* nz, long long* steps) {
+  if (!path) return fail(NFLOW_EINVAL, "null path");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(NFLOW_EINVAL, "cannot open %s", path);
+  CkpHeader h;
+  const int rc = read_header(f, path, h);
+  std::fclose(f);
+  if (rc) return rc;
+  if (nx) *nx = h.nx;
+  if (ny) *ny = h.ny;
+  if (nz) *nz = h.nz;
+  if (steps) *steps = h.steps;
+  return 0;
+}
+
+extern "C" int nflow_save_checkpoint(nflow_solver* s, const char* path) {
+  if (!s || !path) return fail(NFLOW_EINVAL, "nflow_save_checkpoint: null argument");
+  const HostLattice& L = s->b.lats[0];
+  std::vector<float> a(padded_count(L.in));
+  int rc = s->b.export_one(0, a.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  CkpHeader h{};
+  std::memcpy(h.magic, CKP_MAGIC, 8);
+  h.header_bytes = 64;
+  h.version = 1;
+  h.nx = L.in.nx;
+  h.ny = L.in.ny;
+  h.nz = L.in.nz;
+  h.casetype = L.in.casetype;
+  h.steps = L.steps_done;
+  h.storage = L.in.storage;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(NFLOW_EINVAL, "cannot open %s for writing", path);
+  const bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && std::fwrite(a.data(), sizeof(float), a.size(), f) == a.size();
+  const bool closed = std::fclose(f) == 0;
+  return ok && closed ? 0 : fail(NFLOW_EINVAL, "write to %s failed", path);
+}
+
+extern "C" int nflow_load_checkpoint(nflow_solver* s, const char* path) {
+  if (!s || !path) return fail(NFLOW_EINVAL, "nflow_load_checkpoint: null argument");
+  HostLattice& L = s->b.lats[0];
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(NFLOW_EINVAL, "cannot open %s", path);
+  CkpHeader h;
+  int rc = read_header(f, path, h);
+  if (!rc && (h.nx != L.in.nx || h.ny != L.in.ny || h.nz != L.in.nz))
+    rc = fail(NFLOW_EINVAL, "%s holds a %dx%dx%d lattice, the solver is %dx%dx%d", path, h.nx, h.ny, h.nz, L.in.nx,
+              L.in.ny, L.in.nz);
+  std::vector<float> a;
+  if (!rc) {
+    a.resize(padded_count(L.in));
+    if (std::fread(a.data(), sizeof(float), a.size(), f) != a.size())
+      rc = fail(NFLOW_EINVAL, "%s: truncated population data", path);
+  }
+  std::fclose(f);
+  if (rc) return rc;
+  rc = s->b.import_one(0, a.data());
+  if (rc) return rc;
+  L.steps_done = h.steps;  // the run continues from the saved step count
+  return 0;
+}
 
 // ---- batch -----------------------------------------------------------------
 extern "C" int nflow_batch_create(int count, const nflow_input* inputs, nflow_batch** out) {

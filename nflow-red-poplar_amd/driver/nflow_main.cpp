@@ -32,6 +32,9 @@ static void usage() {
       "  --init-rest                    start every node at u = 0 (default: lid nodes at U)\n"
       "  --residual-every K             print and log the residual every K steps (iteration.dat)\n"
       "  --csv                          write result.csv and section.csv at the end\n"
+      "  --vtu FILE                     write the final fields as a VTK unstructured grid\n"
+      "  --restart FILE                 continue from a checkpoint file (same extents)\n"
+      "  --checkpoint FILE              save a checkpoint file at the end\n"
       "  --no-print                     skip the arF slice print\n");
 }
 
@@ -48,6 +51,7 @@ int main(int argc, char** argv) {
   nflow_input_defaults(&in);
   int xd = 3, yd = 3, zd = 3, residual_every = 0;
   bool csv = false, print = true, tau_set = false;
+  std::string vtu, restart, checkpoint;
   double re = 1000.0;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
@@ -78,6 +82,9 @@ int main(int argc, char** argv) {
     else if (a == "--init-rest") in.init_lid_velocity = 0;
     else if (a == "--residual-every") residual_every = std::atoi(next());
     else if (a == "--csv") csv = true;
+    else if (a == "--vtu") vtu = next();
+    else if (a == "--restart") restart = next();
+    else if (a == "--checkpoint") checkpoint = next();
     else if (a == "--no-print") print = false;
     else { usage(); return a == "--help" ? 0 : 2; }
   }
@@ -96,6 +103,10 @@ int main(int argc, char** argv) {
   nflow_solver* s = nullptr;
   check(nflow_create(&in, ntile, sl, &s), "nflow_create");
   check(nflow_initialize(s), "nflow_initialize");
+  if (!restart.empty()) {
+    check(nflow_load_checkpoint(s, restart.c_str()), "nflow_load_checkpoint");
+    std::printf(" - restarted from %s at step %lld\n", restart.c_str(), nflow_steps_done(s));
+  }
 
   FILE* log = nullptr;
   if (residual_every > 0) {
@@ -141,6 +152,16 @@ int main(int argc, char** argv) {
       std::printf("\n");
     }
     std::printf("\n---------------------------------\n");
+  }
+  if (!checkpoint.empty()) check(nflow_save_checkpoint(s, checkpoint.c_str()), "nflow_save_checkpoint");
+  if (!vtu.empty()) {
+    const size_t n = (size_t)nodes;
+    std::vector<float> rho(n), ux(n), uy(n), uz(n);
+    std::vector<int> type(n), dirs(n);
+    check(nflow_read_macro(s, rho.data(), ux.data(), uy.data(), uz.data()), "nflow_read_macro");
+    check(nflow_read_node_type(s, type.data(), dirs.data()), "nflow_read_node_type");
+    check(nflow_write_vtu(vtu.c_str(), in.nx, in.ny, in.nz, rho.data(), ux.data(), uy.data(), uz.data(), type.data()),
+          "nflow_write_vtu");
   }
   if (csv) {
     const size_t n = (size_t)nodes;

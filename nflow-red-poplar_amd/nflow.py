@@ -89,6 +89,11 @@ SIGNATURES = {
     "nflow_timing": (_I, [_VP, _DP, _IP]),
     "nflow_destroy": (None, [_VP]),
     "nflow_last_error": (ctypes.c_char_p, []),
+    "nflow_save_checkpoint": (_I, [_VP, ctypes.c_char_p]),
+    "nflow_load_checkpoint": (_I, [_VP, ctypes.c_char_p]),
+    "nflow_checkpoint_info": (_I, [ctypes.c_char_p, _IP, _IP, _IP, ctypes.POINTER(_LL)]),
+    "nflow_write_vtu": (_I, [ctypes.c_char_p, _I, _I, _I, _FP, _FP, _FP, _FP, _IP]),
+    "nflow_write_csv": (_I, [ctypes.c_char_p, _I, _I, _I, _FP, _FP, _FP, _FP]),
     "nflow_batch_create": (_I, [_I, ctypes.POINTER(Input), ctypes.POINTER(_VP)]),
     "nflow_batch_initialize": (_I, [_VP]),
     "nflow_batch_autotune": (_I, [_VP, _I, ctypes.POINTER(Tuning)]),
@@ -272,6 +277,33 @@ class Solver:
         ms, n = ctypes.c_double(), ctypes.c_int()
         _check(lib().nflow_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def save_checkpoint(self, path):
+        _check(lib().nflow_save_checkpoint(self.h, os.fsencode(path)))
+
+    def load_checkpoint(self, path):
+        _check(lib().nflow_load_checkpoint(self.h, os.fsencode(path)))
+
+
+def checkpoint_info(path):
+    nx, ny, nz, steps = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+    _check(lib().nflow_checkpoint_info(os.fsencode(path), ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz),
+                                       ctypes.byref(steps)))
+    return nx.value, ny.value, nz.value, steps.value
+
+
+def write_vtu(path, rho, ux, uy, uz, node_type=None):
+    ny, nz, nx = rho.shape
+    arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (rho, ux, uy, uz)]
+    t = None if node_type is None else np.ascontiguousarray(node_type, dtype=np.int32)
+    _check(lib().nflow_write_vtu(os.fsencode(path), nx, ny, nz, *[_fp(a) for a in arrs],
+                                 None if t is None else _ip(t)))
+
+
+def write_csv(path, rho, ux, uy, uz):
+    ny, nz, nx = rho.shape
+    arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (rho, ux, uy, uz)]
+    _check(lib().nflow_write_csv(os.fsencode(path), nx, ny, nz, *[_fp(a) for a in arrs]))
 
 
 class Batch:

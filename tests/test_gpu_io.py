@@ -1,0 +1,53 @@
+"""Checkpoint files and output writers on device data."""
+import numpy as np
+import pytest
+
+import nflow
+from helpers import cfg, run_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("storage", [nflow.F32, nflow.F16])
+def test_checkpoint_file_restart(tmp_path, storage):
+    c = cfg(nx=20, ny=16, nz=12, enable_les=1, smagorinsky=0.17, casetype=1)
+    full = run_gpu(c, 70, storage=storage)
+    s = run_gpu(c, 30, storage=storage)
+    p = tmp_path / "state.ckp"
+    s.save_checkpoint(str(p))
+    assert nflow.checkpoint_info(str(p)) == (20, 16, 12, 30)
+    snap = s.read_arF()
+    t = run_gpu(c, 0, storage=storage)
+    t.load_checkpoint(str(p))
+    assert t.steps_done() == 30
+    if storage == nflow.F32:
+        assert np.array_equal(t.read_arF(), snap)  # export straight after a load
+    t.run(40)
+    assert t.steps_done() == 70
+    if storage == nflow.F32:
+        assert np.array_equal(t.read_arF(), full.read_arF())
+    else:  # the import re-rounds the streamed state to FP16
+        a, b = t.read_macro()[1], full.read_macro()[1]
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-3
+
+
+def test_checkpoint_rejects_other_extents(tmp_path):
+    s = run_gpu(cfg(nx=10, ny=10, nz=10), 3)
+    p = tmp_path / "a.ckp"
+    s.save_checkpoint(str(p))
+    t = run_gpu(cfg(nx=12, ny=10, nz=10), 0)
+    with pytest.raises(nflow.NflowError):
+        t.load_checkpoint(str(p))
+
+
+def test_vtu_of_device_fields(tmp_path):
+    s = run_gpu(cfg(nx=9, ny=8, nz=7, init_lid_velocity=1), 5)
+    r, ux, uy, uz = s.read_macro()
+    t, _ = s.node_type()
+    p = tmp_path / "cavity.vtu"
+    nflow.write_vtu(str(p), r, ux, uy, uz, t)
+    from test_io_cpu import parse_vtu
+
+    npts, ncell, a = parse_vtu(p)
+    assert npts == 9 * 8 * 7 and np.array_equal(a["Density"], r.ravel())
+    assert a["Moving Wall"].sum() == (9 - 2) * (7 - 2)
