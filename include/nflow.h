@@ -37,7 +37,8 @@ extern "C" {
 #define NFLOW_YDIM 2
 
 #define NFLOW_CAVITY 1  /* common.h:26 */
-#define NFLOW_CHANNEL 0 /* common.h:27 (POISEUILLE); here a square duct */
+#define NFLOW_CHANNEL 0 /* square duct: parabolic velocity inlet, pressure outlet, walls */
+#define NFLOW_PIPE 2    /* common.h:27 POISEUILLE: circular pipe of solid nodes (kernel.cu:1078-1095) */
 
 #define NFLOW_NODE_FLUID (1 << 1) /* common.h:32-37 */
 #define NFLOW_NODE_FIXEDWALL (1 << 2)
@@ -78,6 +79,9 @@ typedef struct nflow_input {
   double inlet_peak;     /* channel: peak inlet velocity of the parabola */
   double perturb;        /* channel: relative amplitude of the seeded inlet perturbation */
   unsigned long long seed;
+  /* pipe: nodes with (y - cy)^2 + (z - cz)^2 > r^2 are solid (kernel.cu:1090
+   * uses cy = cz = 25, r = 20); the inlet velocity is ref_vel (kernel.cu:601) */
+  double pipe_radius, pipe_cy, pipe_cz;
 } nflow_input;
 
 typedef struct nflow_solver nflow_solver;
@@ -126,6 +130,11 @@ int nflow_write_arF(nflow_solver* s, const float* src);
 int nflow_read_macro(nflow_solver* s, float* rho, float* ux, float* uy, float* uz);
 /* Node flags and direction masks (keSetNodeType / keInitialNodeDirection), [ny][nz][nx]. */
 int nflow_read_node_type(nflow_solver* s, int* node_type, int* direction_state);
+/* Solid obstacles (NODE_STRUCT, kernel.cu:659-697, 1090): mask[ny][nz][nx],
+ * non-zero = solid.  Solid nodes are not updated and read back as zero; a
+ * link whose donor is solid bounces back at the node.  Call before
+ * nflow_initialize (the pipe case builds its own mask). */
+int nflow_set_solid_mask(nflow_solver* s, const unsigned char* mask);
 /* FP64 reductions on the device: total mass, momentum and the residual
  * sum |PrevDen - CurrDen| of kernel.cu:241-245. */
 int nflow_diagnostics(nflow_solver* s, double* mass, double mom[3], double* drho_l1);

@@ -21,6 +21,7 @@ struct HostLattice {
   size_t bytes = 0;      // device bytes of this lattice
   void* mem = nullptr;   // base of its device memory
   bool own_mem = false;  // allocated with hipMalloc by this lattice
+  void* mask_mem = nullptr;  // solid mask allocated by nflow_set_solid_mask (pipe: inside mem)
   LatticeDev dev{};
   long long steps_done = 0;
   int cur = 0;  // buffer holding the newest post-collision state
@@ -34,8 +35,11 @@ int validate_input(const nflow_input& in, int storage);
 size_t lattice_bytes(const nflow_input& in, int storage);
 // Lay the lattice's buffers out in `mem` (lattice_bytes long) and fill dev.
 void lattice_layout(HostLattice& h, const nflow_input& in, int storage, void* mem);
-// Upload the channel inlet profile (host-computed, double -> float).
+// Upload the inlet profile of a channel or pipe (host-computed, double ->
+// float) and the pipe's solid mask.
 int lattice_upload_inlet(HostLattice& h, hipStream_t st);
+// Pipe geometry: node (x, y, z) is solid when (y - cy)^2 + (z - cz)^2 > r^2.
+bool pipe_solid(const nflow_input& in, int y, int z);
 
 class Batch {
  public:
@@ -53,6 +57,7 @@ class Batch {
   int nitems = 0;
   int grid = 0;
   bool tables_dirty = true;
+  bool masked = false;  // some lattice has solid nodes: step with the masked kernel
 
   int gparity = 0;  // parity argument of the next step launch
   hipGraphExec_t graph[2] = {nullptr, nullptr};

@@ -163,22 +163,28 @@ __device__ inline int node_type(const LatticeDev& L, int x, int y, int z) {
 // Streaming + boundary rules: f_q(x) for the step just completed.
 // Donor inside: f_q(x) = f*_q(x - e_q).  Donor outside (the link x -> x + e_opp
 // leaves the box): the rule of the node's type applied to f*_opp(q)(x).
+// MASK: a solid donor bounces back, f_q(x) = f*_opp(q)(x) (keBoundaryStruct,
+// kernel.cu:682-695, applied here to every non-solid node).
 // ---------------------------------------------------------------------------
-template <class St>
+template <class St, bool MASK = false>
 __device__ inline void pull(const LatticeDev& L, const typename St::T* __restrict__ fin, uint32_t n, int x, int y,
                             int z, float f[Q]) {
   const bool xlo = x == 0, xhi = x == L.nx - 1;
   const bool ylo = y == 0, yhi = y == L.ny - 1;
   const bool zlo = z == 0, zhi = z == L.nz - 1;
   const uint32_t ps = (uint32_t)L.pstride;
+  const gptr<const uint8_t> mk = (gptr<const uint8_t>)L.mask;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const bool out = (EX[q] > 0 && xlo) || (EX[q] < 0 && xhi) || (EY[q] > 0 && ylo) || (EY[q] < 0 && yhi) ||
-                     (EZ[q] > 0 && zlo) || (EZ[q] < 0 && zhi);
+    bool out = (EX[q] > 0 && xlo) || (EX[q] < 0 && xhi) || (EY[q] > 0 && ylo) || (EY[q] < 0 && yhi) ||
+               (EZ[q] > 0 && zlo) || (EZ[q] < 0 && zhi);
     // (a select of two full index computations: measured faster than
     // selecting the source node and the plane separately, which needs more
     // registers)
     const int off = EX[q] + EZ[q] * L.nx + EY[q] * L.nxz;
+    if (MASK && q != 0) {
+      if (mk && !out) out = mk[(int)n - off] != 0;
+    }
     const uint32_t idx =
         out ? pidx<typename St::T>(OPP[q], n, ps) : pidx<typename St::T>(q, (uint32_t)((int)n - off), ps);
     f[q] = St::load(fin, idx) + shift_of<St>(q);
@@ -224,11 +230,15 @@ __device__ inline void boundary(const LatticeDev& L, const float* __restrict__ s
   }
 }
 
-template <class St>
+template <class St, bool MASK = false>
 __device__ inline void gather(const LatticeDev& L, const typename St::T* __restrict__ fin,
                               const float* __restrict__ side, uint32_t n, int x, int y, int z, float f[Q]) {
-  pull<St>(L, fin, n, x, y, z, f);
+  pull<St, MASK>(L, fin, n, x, y, z, f);
   boundary(L, side, x, y, z, f);
+}
+
+__device__ inline bool is_solid(const LatticeDev& L, uint32_t n) {
+  return L.mask && ((gptr<const uint8_t>)L.mask)[n] != 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -413,7 +423,7 @@ __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], typename
   float rho, jx, jy, jz;
   moments(f, rho, jx, jy, jz);
   const float ir = 1.0f / rho;
-  if (L.casetype == CASE_CHANNEL) {
+  if (L.casetype != CASE_CAVITY) {  // channel and pipe: inlet / outlet side planes
     const int pl = L.ny * L.nz, yz = y * L.nz + z;
     const gptr<float> so = (gptr<float>)side_out;
     if (x == 1) so[SIDE_RHO_IN * pl + yz] = rho;
@@ -458,7 +468,9 @@ __device__ inline void coords(const LatticeDev& L, uint32_t n, int& x, int& y, i
 #define NFLOW_STEP_ATTR
 #endif
 
-template <class St, int BS>
+// MASK = true: at least one lattice of the launch has solid nodes (the plain
+// instantiation carries no mask code at all).
+template <class St, int BS, bool MASK>
 __global__ void __launch_bounds__(BS) NFLOW_STEP_ATTR k_step(const LatticeDev* __restrict__ lats, const WorkItem* __restrict__ items,
                                              int nitems, int parity) {
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
@@ -466,11 +478,12 @@ __global__ void __launch_bounds__(BS) NFLOW_STEP_ATTR k_step(const LatticeDev* _
     if (threadIdx.x < wi.count) {
       const LatticeDev& L = lats[wi.lat];
       const uint32_t n = wi.start + threadIdx.x;
+      if (MASK && is_solid(L, n)) continue;  // solid nodes are never updated
       const int p = parity ^ L.parity0;
       int x, y, z;
       coords(L, n, x, y, z);
       float f[Q];
-      gather<St>(L, static_cast<const typename St::T*>(L.buf[p]), L.side[p], n, x, y, z, f);
+      gather<St, MASK>(L, static_cast<const typename St::T*>(L.buf[p]), L.side[p], n, x, y, z, f);
       relax_and_store<St>(L, f, static_cast<typename St::T*>(L.buf[p ^ 1]), L.side[p ^ 1], n, x, y, z);
     }
   }
@@ -485,6 +498,28 @@ __global__ void __launch_bounds__(256) k_init(LatticeDev L, int cur, const float
   if (n >= L.n) return;
   int x, y, z;
   coords(L, n, x, y, z);
+  if (is_solid(L, n)) {
+    // solid: zero populations in both buffers (keInitialize skips them,
+    // kernel.cu:294, leaving the memset zeros) and rho = u = 0 in both side
+    // buffers, never touched again
+    const uint32_t ps = (uint32_t)L.pstride;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      St::store(static_cast<typename St::T*>(L.buf[0]), pidx<typename St::T>(q, n, ps), -shift_of<St>(q));
+      St::store(static_cast<typename St::T*>(L.buf[1]), pidx<typename St::T>(q, n, ps), -shift_of<St>(q));
+    }
+    if (L.casetype != CASE_CAVITY && (x == 1 || x >= L.nx - 2)) {
+      const int pl = L.ny * L.nz, yz = y * L.nz + z;
+      for (int b = 0; b < 2; ++b) {
+        const gptr<float> so = (gptr<float>)L.side[b];
+        if (x == 1) so[SIDE_RHO_IN * pl + yz] = 0.0f;
+        const int base = x == L.nx - 1 ? SIDE_U_OUT : SIDE_U_OUT1;
+        if (x >= L.nx - 2)
+          for (int c = 0; c < 3; ++c) so[(base + c) * pl + yz] = 0.0f;
+      }
+    }
+    return;
+  }
   float f[Q];
   if (src) {
     const size_t b = (((size_t)(y + 1) * (L.nz + 2) + (z + 1)) * (L.nx + 2) + (x + 1)) * Q;
@@ -539,7 +574,15 @@ __global__ void __launch_bounds__(EXPORT_BS) k_export(LatticeDev L, int prev, in
   __shared__ double sh[EXPORT_BS / 64];
   const uint32_t n = blockIdx.x * (uint32_t)EXPORT_BS + threadIdx.x;
   double dm = 0.0, dx = 0.0, dy = 0.0, dz = 0.0, dr = 0.0;
-  if (n < L.n) {
+  if (n < L.n && is_solid(L, n)) {  // solid nodes read back as zeros, outside the sums
+    int x, y, z;
+    coords(L, n, x, y, z);
+    if (aos) {
+      const size_t b = (((size_t)(y + 1) * (L.nz + 2) + (z + 1)) * (L.nx + 2) + (x + 1)) * Q;
+      for (int q = 0; q < Q; ++q) aos[b + q] = 0.0f;
+    }
+    if (rho_o) rho_o[n] = ux_o[n] = uy_o[n] = uz_o[n] = 0.0f;
+  } else if (n < L.n) {
     int x, y, z;
     coords(L, n, x, y, z);
     const typename St::T* pb = static_cast<const typename St::T*>(L.buf[prev]);
@@ -549,7 +592,7 @@ __global__ void __launch_bounds__(EXPORT_BS) k_export(LatticeDev L, int prev, in
 #pragma unroll
       for (int q = 0; q < Q; ++q) f[q] = St::load(pb, pidx<typename St::T>(q, n, ps)) + shift_of<St>(q);
     } else {
-      gather<St>(L, pb, L.side[prev], n, x, y, z, f);
+      gather<St, true>(L, pb, L.side[prev], n, x, y, z, f);
     }
     float rho, jx, jy, jz;
     moments(f, rho, jx, jy, jz);
@@ -627,7 +670,7 @@ __global__ void __launch_bounds__(256) k_node_type(LatticeDev L, int* __restrict
     const int x1 = x + EX[q], y1 = y + EY[q], z1 = z + EZ[q];
     if (x1 >= 0 && x1 < L.nx && y1 >= 0 && y1 < L.ny && z1 >= 0 && z1 < L.nz) ds |= 1 << q;
   }
-  type[n] = node_type(L, x, y, z);
+  type[n] = is_solid(L, n) ? NODE_STRUCT : node_type(L, x, y, z);
   dirstate[n] = ds;
 }
 
@@ -644,26 +687,41 @@ __global__ void __launch_bounds__(256) k_copy_streams(const float4* __restrict__
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <class St>
+template <class St, bool MASK>
 static hipError_t step_bs(int block, int grid, hipStream_t st, const LatticeDev* d_lats, const WorkItem* d_items,
                           int nitems, int parity) {
   switch (block) {
-    case 64: hipLaunchKernelGGL((k_step<St, 64>), dim3(grid), dim3(64), 0, st, d_lats, d_items, nitems, parity); break;
-    case 128: hipLaunchKernelGGL((k_step<St, 128>), dim3(grid), dim3(128), 0, st, d_lats, d_items, nitems, parity); break;
-    case 256: hipLaunchKernelGGL((k_step<St, 256>), dim3(grid), dim3(256), 0, st, d_lats, d_items, nitems, parity); break;
-    case 512: hipLaunchKernelGGL((k_step<St, 512>), dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity); break;
+    case 64:
+      hipLaunchKernelGGL((k_step<St, 64, MASK>), dim3(grid), dim3(64), 0, st, d_lats, d_items, nitems, parity);
+      break;
+    case 128:
+      hipLaunchKernelGGL((k_step<St, 128, MASK>), dim3(grid), dim3(128), 0, st, d_lats, d_items, nitems, parity);
+      break;
+    case 256:
+      hipLaunchKernelGGL((k_step<St, 256, MASK>), dim3(grid), dim3(256), 0, st, d_lats, d_items, nitems, parity);
+      break;
+    case 512:
+      hipLaunchKernelGGL((k_step<St, 512, MASK>), dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_step(int storage, int block, int grid, hipStream_t st, const LatticeDev* d_lats,
+template <class St>
+static hipError_t step_mask(bool masked, int block, int grid, hipStream_t st, const LatticeDev* d_lats,
+                            const WorkItem* d_items, int nitems, int parity) {
+  return masked ? step_bs<St, true>(block, grid, st, d_lats, d_items, nitems, parity)
+                : step_bs<St, false>(block, grid, st, d_lats, d_items, nitems, parity);
+}
+
+hipError_t launch_step(int storage, bool masked, int block, int grid, hipStream_t st, const LatticeDev* d_lats,
                        const WorkItem* d_items, int nitems, int parity) {
   if (grid <= 0 || nitems <= 0) return hipSuccess;
   switch (storage) {
-    case ST_F32: return step_bs<StF32>(block, grid, st, d_lats, d_items, nitems, parity);
-    case ST_F16: return step_bs<StF16>(block, grid, st, d_lats, d_items, nitems, parity);
-    case ST_BF16: return step_bs<StBF16>(block, grid, st, d_lats, d_items, nitems, parity);
+    case ST_F32: return step_mask<StF32>(masked, block, grid, st, d_lats, d_items, nitems, parity);
+    case ST_F16: return step_mask<StF16>(masked, block, grid, st, d_lats, d_items, nitems, parity);
+    case ST_BF16: return step_mask<StBF16>(masked, block, grid, st, d_lats, d_items, nitems, parity);
   }
   return hipErrorInvalidValue;
 }

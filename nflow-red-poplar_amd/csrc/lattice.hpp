@@ -6,7 +6,7 @@
 
 namespace nflow {
 
-enum : int { CASE_CHANNEL = 0, CASE_CAVITY = 1 };
+enum : int { CASE_CHANNEL = 0, CASE_CAVITY = 1, CASE_PIPE = 2 };
 enum : int { MODEL_SRT = 0, MODEL_MRT = 1 };
 enum : int { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2 };
 enum : int {
@@ -52,7 +52,8 @@ inline FastDiv make_fastdiv(uint32_t d) {
 struct alignas(16) LatticeDev {
   void* buf[2];
   float* side[2];
-  const float* uin;   // channel inlet profile [ny][nz]
+  const float* uin;   // inlet velocity profile [ny][nz] (channel, pipe)
+  const uint8_t* mask;  // solid nodes (non-zero), or null
   long long pstride;  // elements between population planes
   int nx, ny, nz, nxz;
   uint32_t n;         // interior nodes
@@ -73,7 +74,9 @@ struct alignas(16) WorkItem {
 };
 
 // Kernel launchers (kernels.hip).
-hipError_t launch_step(int storage, int block, int grid, hipStream_t st, const LatticeDev* d_lats,
+// masked: use the instantiation that honours LatticeDev::mask (any lattice of
+// the launch has solid nodes).
+hipError_t launch_step(int storage, bool masked, int block, int grid, hipStream_t st, const LatticeDev* d_lats,
                        const WorkItem* d_items, int nitems, int parity);
 hipError_t launch_init(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src_aos);
 hipError_t launch_export(int storage, hipStream_t st, const LatticeDev& L, int prev, int prev_streamed, float* aos,

@@ -28,9 +28,11 @@ static size_t elem_size(int storage) { return storage == NFLOW_STORAGE_F32 ? 4 :
 
 int validate_input(const nflow_input& in, int storage) {
   if (in.nx < 1 || in.ny < 1 || in.nz < 1) return fail(NFLOW_EINVAL, "lattice extents must be >= 1");
-  if (in.casetype != NFLOW_CAVITY && in.casetype != NFLOW_CHANNEL)
-    return fail(NFLOW_EINVAL, "casetype must be NFLOW_CAVITY or NFLOW_CHANNEL");
-  if (in.casetype == NFLOW_CHANNEL && in.nx < 3) return fail(NFLOW_EINVAL, "channel needs nx >= 3");
+  if (in.casetype != NFLOW_CAVITY && in.casetype != NFLOW_CHANNEL && in.casetype != NFLOW_PIPE)
+    return fail(NFLOW_EINVAL, "casetype must be NFLOW_CAVITY, NFLOW_CHANNEL or NFLOW_PIPE");
+  if (in.casetype != NFLOW_CAVITY && in.nx < 3) return fail(NFLOW_EINVAL, "channel and pipe need nx >= 3");
+  if (in.casetype == NFLOW_PIPE && !(in.pipe_radius > 0.0))
+    return fail(NFLOW_EINVAL, "pipe_radius must be positive (got %g)", in.pipe_radius);
   if (storage < NFLOW_STORAGE_F32 || storage > NFLOW_STORAGE_BF16) return fail(NFLOW_EINVAL, "bad storage");
   const unsigned long long n = (unsigned long long)in.nx * in.ny * in.nz;
   const unsigned long long pstride = round_up(n, 128) + NFLOW_PLANE_PAD;
@@ -42,7 +44,7 @@ int validate_input(const nflow_input& in, int storage) {
 }
 
 struct Layout {
-  size_t pstride, pop, side, uin, total;
+  size_t pstride, pop, side, uin, mask, total;
 };
 
 static Layout layout_of(const nflow_input& in, int storage) {
@@ -52,8 +54,14 @@ static Layout layout_of(const nflow_input& in, int storage) {
   l.pop = round_up(19 * l.pstride * elem_size(storage), 256);
   l.side = round_up((size_t)SIDE_PLANES * in.ny * in.nz * sizeof(float), 256);
   l.uin = round_up((size_t)in.ny * in.nz * sizeof(float), 256);
-  l.total = 2 * l.pop + 2 * l.side + l.uin;
+  l.mask = in.casetype == NFLOW_PIPE ? round_up(n, 256) : 0;  // one byte per node
+  l.total = 2 * l.pop + 2 * l.side + l.uin + l.mask;
   return l;
+}
+
+bool pipe_solid(const nflow_input& in, int y, int z) {
+  const double dy = y - in.pipe_cy, dz = z - in.pipe_cz;
+  return dy * dy + dz * dz > in.pipe_radius * in.pipe_radius;
 }
 
 size_t lattice_bytes(const nflow_input& in, int storage) { return layout_of(in, storage).total; }
@@ -72,6 +80,7 @@ void lattice_layout(HostLattice& h, const nflow_input& in, int storage, void* me
   d.side[0] = reinterpret_cast<float*>(p + 2 * l.pop);
   d.side[1] = reinterpret_cast<float*>(p + 2 * l.pop + l.side);
   d.uin = reinterpret_cast<const float*>(p + 2 * l.pop + 2 * l.side);
+  d.mask = l.mask ? reinterpret_cast<const uint8_t*>(p + 2 * l.pop + 2 * l.side + l.uin) : nullptr;
   d.pstride = (long long)l.pstride;
   d.nx = in.nx;
   d.ny = in.ny;
@@ -80,7 +89,7 @@ void lattice_layout(HostLattice& h, const nflow_input& in, int storage, void* me
   d.n = (uint32_t)((size_t)in.nx * in.ny * in.nz);
   d.div_nx = make_fastdiv((uint32_t)in.nx);
   d.div_nxz = make_fastdiv((uint32_t)d.nxz);
-  d.casetype = in.casetype == NFLOW_CAVITY ? CASE_CAVITY : CASE_CHANNEL;
+  d.casetype = in.casetype == NFLOW_CAVITY ? CASE_CAVITY : in.casetype == NFLOW_PIPE ? CASE_PIPE : CASE_CHANNEL;
   d.model = in.enable_mrt ? MODEL_MRT : MODEL_SRT;
   // with Cs = 0 the Smagorinsky term vanishes identically (tau_eddy = 0)
   d.les = (in.enable_les && in.smagorinsky > 0.0) ? 1 : 0;
@@ -109,10 +118,14 @@ static uint64_t splitmix64(uint64_t x) {
 
 int lattice_upload_inlet(HostLattice& h, hipStream_t st) {
   const nflow_input& in = h.in;
-  if (in.casetype != NFLOW_CHANNEL) return 0;
+  if (in.casetype == NFLOW_CAVITY) return 0;
   std::vector<float> u((size_t)in.ny * in.nz);
   for (int y = 0; y < in.ny; ++y)
     for (int z = 0; z < in.nz; ++z) {
+      if (in.casetype == NFLOW_PIPE) {  // keBoundaryInlet, kernel.cu:599-608: uniform RefVelx
+        u[(size_t)y * in.nz + z] = (float)in.ref_vel[0];
+        continue;
+      }
       const double eta = (y + 0.5) / in.ny, zeta = (z + 0.5) / in.nz;
       const uint64_t key = (uint64_t)in.seed ^ (((uint64_t)(uint32_t)y << 32) | (uint64_t)(uint32_t)z);
       const double xi = 2.0 * ((double)(splitmix64(key) >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
@@ -120,6 +133,16 @@ int lattice_upload_inlet(HostLattice& h, hipStream_t st) {
           (float)(in.inlet_peak * 16.0 * eta * (1.0 - eta) * zeta * (1.0 - zeta) * (1.0 + in.perturb * xi));
     }
   NFLOW_HIP(hipMemcpyAsync(const_cast<float*>(h.dev.uin), u.data(), u.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  std::vector<uint8_t> m;
+  if (in.casetype == NFLOW_PIPE) {  // keSetNodeType, kernel.cu:1078-1094
+    m.resize((size_t)h.dev.n);
+    for (int y = 0; y < in.ny; ++y)
+      for (int z = 0; z < in.nz; ++z) {
+        const uint8_t s = pipe_solid(in, y, z) ? 1 : 0;
+        std::memset(&m[(size_t)in.nx * ((size_t)z + (size_t)in.nz * y)], s, (size_t)in.nx);
+      }
+    NFLOW_HIP(hipMemcpyAsync(const_cast<uint8_t*>(h.dev.mask), m.data(), m.size(), hipMemcpyHostToDevice, st));
+  }
   NFLOW_HIP(hipStreamSynchronize(st));
   return 0;
 }
@@ -128,8 +151,10 @@ int lattice_upload_inlet(HostLattice& h, hipStream_t st) {
 Batch::~Batch() {
   if (stream) (void)hipStreamSynchronize(stream);
   destroy_graphs();
-  for (auto& h : lats)
+  for (auto& h : lats) {
     if (h.own_mem && h.mem) (void)hipFree(h.mem);
+    if (h.mask_mem) (void)hipFree(h.mask_mem);
+  }
   (void)hipFree(d_lats);
   (void)hipFree(d_items);
   (void)hipFree(d_aos);
@@ -181,6 +206,7 @@ int Batch::add_lattice(const nflow_input& in, const std::vector<Tile>& tiles, vo
 void Batch::remove_lattice(int idx) {
   if (idx < 0 || idx >= (int)lats.size()) return;
   if (lats[idx].own_mem && lats[idx].mem) (void)hipFree(lats[idx].mem);
+  if (lats[idx].mask_mem) (void)hipFree(lats[idx].mask_mem);
   lats.erase(lats.begin() + idx);
   tables_dirty = true;
 }
@@ -229,9 +255,11 @@ int Batch::upload_tables() {
   const int bs = tuning.block;  // nodes per work item
   std::vector<WorkItem> items;
   std::vector<LatticeDev> devs(lats.size());
+  masked = false;
   for (size_t li = 0; li < lats.size(); ++li) {
     const HostLattice& h = lats[li];
     devs[li] = h.dev;
+    if (h.dev.mask) masked = true;
     const uint32_t nx = (uint32_t)h.in.nx, nxz = (uint32_t)h.in.nx * (uint32_t)h.in.nz;
     auto emit = [&](uint32_t a, uint32_t b) {
       for (uint32_t s = a; s < b; s += (uint32_t)bs)
@@ -311,7 +339,7 @@ int Batch::build_graph(int parity) {
   int p = parity;
   hipError_t err = hipSuccess;
   for (int s = 0; s < tuning.graph_steps && err == hipSuccess; ++s) {
-    err = launch_step(storage, tuning.block, grid, stream, d_lats, d_items, nitems, p);
+    err = launch_step(storage, masked, tuning.block, grid, stream, d_lats, d_items, nitems, p);
     p ^= 1;
   }
   hipError_t e2 = hipStreamEndCapture(stream, &g);
@@ -346,7 +374,7 @@ int Batch::launch_steps(int nsteps) {
     launches += (long long)reps * gs;
   }
   for (int s = 0; s < left; ++s) {
-    NFLOW_HIP(launch_step(storage, tuning.block, grid, stream, d_lats, d_items, nitems, gparity));
+    NFLOW_HIP(launch_step(storage, masked, tuning.block, grid, stream, d_lats, d_items, nitems, gparity));
     gparity ^= 1;
     ++launches;
   }
@@ -618,6 +646,33 @@ extern "C" int nflow_read_macro(nflow_solver* s, float* rho, float* ux, float* u
 extern "C" int nflow_read_node_type(nflow_solver* s, int* node_type, int* direction_state) {
   if (!s || !node_type || !direction_state) return fail(NFLOW_EINVAL, "nflow_read_node_type: null argument");
   return s->b.node_types(0, node_type, direction_state);
+}
+
+extern "C" int nflow_set_solid_mask(nflow_solver* s, const unsigned char* mask) {
+  if (!s) return fail(NFLOW_EINVAL, "null solver");
+  Batch& b = s->b;
+  HostLattice& h = b.lats[0];
+  if (h.initialized) return fail(NFLOW_ESTATE, "nflow_set_solid_mask: call before nflow_initialize");
+  NFLOW_HIP(hipStreamSynchronize(b.stream));
+  b.tables_dirty = true;
+  if (!mask) {  // back to the case's own geometry
+    if (h.mask_mem) {
+      (void)hipFree(h.mask_mem);
+      h.mask_mem = nullptr;
+      h.dev.mask = nullptr;
+    }
+    return h.in.casetype == NFLOW_PIPE ? lattice_upload_inlet(h, b.stream) : 0;
+  }
+  if (!h.dev.mask) {
+    if (hipMalloc(&h.mask_mem, (size_t)h.dev.n) != hipSuccess) {
+      (void)hipGetLastError();
+      h.mask_mem = nullptr;
+      return fail(NFLOW_ENOMEM, "hipMalloc of a %u-byte solid mask failed", h.dev.n);
+    }
+    h.dev.mask = static_cast<const uint8_t*>(h.mask_mem);
+  }
+  NFLOW_HIP(hipMemcpy(const_cast<uint8_t*>(h.dev.mask), mask, (size_t)h.dev.n, hipMemcpyHostToDevice));
+  return 0;
 }
 
 extern "C" int nflow_diagnostics(nflow_solver* s, double* mass, double mom[3], double* drho_l1) {

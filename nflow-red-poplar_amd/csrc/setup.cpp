@@ -32,6 +32,9 @@ extern "C" void nflow_input_defaults(nflow_input* in) {
   in->inlet_peak = 0.05;
   in->perturb = 0.01;
   in->seed = 0;
+  in->pipe_radius = 20.0;  // kernel.cu:1090: (y-25)^2 + (z-25)^2 > 400
+  in->pipe_cy = 25.0;
+  in->pipe_cz = 25.0;
 }
 
 // decomposition.cpp:98-161: `dec` equal blocks of n/dec nodes per axis, the

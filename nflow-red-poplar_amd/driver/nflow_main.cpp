@@ -28,6 +28,7 @@ static void usage() {
       "  --no-les                       disable the LES pass\n"
       "  --storage f32|f16|bf16         population storage (default f32)\n"
       "  --channel                      square-duct channel instead of the cavity\n"
+      "  --pipe [R CY CZ]               circular pipe (solid outside radius R about (CY, CZ); default 20 25 25)\n"
       "  --decomp X Y Z                 sub-block counts (default 3 3 3, common.h:17-19)\n"
       "  --init-rest                    start every node at u = 0 (default: lid nodes at U)\n"
       "  --residual-every K             print and log the residual every K steps (iteration.dat)\n"
@@ -78,6 +79,14 @@ int main(int argc, char** argv) {
       std::string s = next();
       in.storage = s == "f16" ? NFLOW_STORAGE_F16 : s == "bf16" ? NFLOW_STORAGE_BF16 : NFLOW_STORAGE_F32;
     } else if (a == "--channel") in.casetype = NFLOW_CHANNEL;
+    else if (a == "--pipe") {
+      in.casetype = NFLOW_PIPE;
+      if (i + 3 < argc && argv[i + 1][0] != '-') {
+        in.pipe_radius = std::atof(next());
+        in.pipe_cy = std::atof(next());
+        in.pipe_cz = std::atof(next());
+      }
+    }
     else if (a == "--decomp") { xd = std::atoi(next()); yd = std::atoi(next()); zd = std::atoi(next()); }
     else if (a == "--init-rest") in.init_lid_velocity = 0;
     else if (a == "--residual-every") residual_every = std::atoi(next());
@@ -92,7 +101,7 @@ int main(int argc, char** argv) {
   if (!tau_set) in.tau = 3.0 * in.ref_vel[0] * in.h / re + 0.5;  // main.cpp:81
 
   std::printf(" - %dx%dx%d %s, %s%s, tau = %.6f, Cs = %.3f, storage = %d\n", in.nx, in.ny, in.nz,
-              in.casetype == NFLOW_CAVITY ? "cavity" : "channel", in.enable_mrt ? "MRT" : "SRT",
+              in.casetype == NFLOW_CAVITY ? "cavity" : in.casetype == NFLOW_PIPE ? "pipe" : "channel", in.enable_mrt ? "MRT" : "SRT",
               in.enable_les ? "+LES" : "", in.tau, in.smagorinsky, in.storage);
 
   std::vector<int> slice(6 * (size_t)xd * yd * zd);

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libnflow.so")
 Q = 19
-CAVITY, CHANNEL = 1, 0
+CAVITY, CHANNEL, PIPE = 1, 0, 2
 F32, F16, BF16 = 0, 1, 2
 NODE_FLUID, NODE_FIXEDWALL, NODE_MOVINGWALL = 1 << 1, 1 << 2, 1 << 3
 NODE_INLET, NODE_OUTLET, NODE_STRUCT = 1 << 4, 1 << 5, 1 << 6
@@ -35,6 +35,7 @@ class Input(ctypes.Structure):
         ("ref_vel", ctypes.c_double * 3),
         ("w_eps", ctypes.c_double), ("w_epsj", ctypes.c_double), ("w_xx", ctypes.c_double),
         ("inlet_peak", ctypes.c_double), ("perturb", ctypes.c_double), ("seed", ctypes.c_ulonglong),
+        ("pipe_radius", ctypes.c_double), ("pipe_cy", ctypes.c_double), ("pipe_cz", ctypes.c_double),
     ]
 
 
@@ -85,6 +86,7 @@ SIGNATURES = {
     "nflow_write_arF": (_I, [_VP, _FP]),
     "nflow_read_macro": (_I, [_VP, _FP, _FP, _FP, _FP]),
     "nflow_read_node_type": (_I, [_VP, _IP, _IP]),
+    "nflow_set_solid_mask": (_I, [_VP, ctypes.c_void_p]),
     "nflow_diagnostics": (_I, [_VP, _DP, _DP, _DP]),
     "nflow_timing": (_I, [_VP, _DP, _IP]),
     "nflow_destroy": (None, [_VP]),
@@ -266,6 +268,17 @@ class Solver:
         d = np.empty(self.shape, dtype=np.int32)
         _check(lib().nflow_read_node_type(self.h, _ip(t), _ip(d)))
         return t, d
+
+    def set_solid_mask(self, mask):
+        """Solid nodes, [ny][nz][nx] booleans (None: the case's own geometry).
+        Call before initialize()."""
+        if mask is None:
+            _check(lib().nflow_set_solid_mask(self.h, None))
+            return
+        m = np.ascontiguousarray(np.asarray(mask) != 0, dtype=np.uint8)
+        if m.shape != self.shape:
+            raise ValueError(f"mask must have shape {self.shape}")
+        _check(lib().nflow_set_solid_mask(self.h, m.ctypes.data))
 
     def diagnostics(self):
         m, d = ctypes.c_double(), ctypes.c_double()

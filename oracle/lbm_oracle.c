@@ -134,6 +134,9 @@ void orc_default_params(orc_params* p) {
   p->inlet_peak = 0.05;
   p->perturb = 0.01;
   p->seed = 0;
+  p->pipe_radius = 20.0; /* kernel.cu:1090: (y-25)^2 + (z-25)^2 > 400 */
+  p->pipe_cy = 25.0;
+  p->pipe_cz = 25.0;
   static const double S[ORC_Q] = {0.0, 1.19, 1.4, 0.0,  1.2, 0.0,  1.2, 0.0,  1.2, -1.0,
                                   1.4, -1.0, 1.4, -1.0, -1.0, -1.0, 1.98, 1.98, 1.98};
   memcpy(p->s_rates, S, sizeof(S));
@@ -363,6 +366,7 @@ struct orc_lattice {
   real *rho, *prev_rho, *ux, *uy, *uz, *tau_eddy;
   int *type, *dirstate;
   real* uin; /* [ny][nz] */
+  unsigned char* mask; /* [ny][nz][nx], non-zero = solid; NULL: none */
   long steps;
 };
 
@@ -375,7 +379,7 @@ static inline size_t pid(const orc_lattice* L, int X, int Y, int Z) { /* padded 
 
 orc_lattice* orc_create(const orc_params* p, int ntile, const int slice[][3][2]) {
   if (p->nx < 1 || p->ny < 1 || p->nz < 1 || ntile < 1) return NULL;
-  if (p->casetype == ORC_CHANNEL && p->nx < 3) return NULL;
+  if (p->casetype != ORC_CAVITY && p->nx < 3) return NULL;
   orc_lattice* L = (orc_lattice*)calloc(1, sizeof(orc_lattice));
   L->p = *p;
   L->ntile = ntile;
@@ -404,15 +408,38 @@ orc_lattice* orc_create(const orc_params* p, int ntile, const int slice[][3][2])
     return NULL;
   }
   orc_inlet_profile(p, uin);
-  for (size_t i = 0; i < (size_t)p->ny * p->nz; ++i) L->uin[i] = (real)uin[i];
+  for (size_t i = 0; i < (size_t)p->ny * p->nz; ++i)
+    L->uin[i] = p->casetype == ORC_PIPE ? (real)p->ref_vel[0] : (real)uin[i]; /* pipe: kernel.cu:599-608 */
   free(uin);
+  orc_set_mask(L, NULL);
   return L;
+}
+
+static int pipe_solid(const orc_params* p, int y, int z) {
+  const double dy = y - p->pipe_cy, dz = z - p->pipe_cz;
+  return dy * dy + dz * dz > p->pipe_radius * p->pipe_radius;
+}
+
+void orc_set_mask(orc_lattice* L, const unsigned char* mask) {
+  const orc_params* p = &L->p;
+  free(L->mask);
+  L->mask = NULL;
+  if (!mask && p->casetype != ORC_PIPE) return;
+  L->mask = (unsigned char*)malloc(L->n);
+  if (mask) {
+    memcpy(L->mask, mask, L->n);
+    return;
+  }
+  for (int y = 0; y < p->ny; ++y) /* keSetNodeType, kernel.cu:1078-1094 */
+    for (int z = 0; z < p->nz; ++z)
+      for (int x = 0; x < p->nx; ++x) L->mask[nid(L, x, y, z)] = (unsigned char)pipe_solid(p, y, z);
 }
 
 void orc_destroy(orc_lattice* L) {
   if (!L) return;
   free(L->slice); free(L->move); free(L->arF); free(L->arFTemp); free(L->rho); free(L->prev_rho);
   free(L->ux); free(L->uy); free(L->uz); free(L->tau_eddy); free(L->type); free(L->dirstate); free(L->uin);
+  free(L->mask);
   free(L);
 }
 
@@ -429,7 +456,8 @@ int orc_num_threads(void) {
 /* Node type for global coordinates.  Cavity: lbm_codelets.cpp:183-201.
  * Channel (square duct, no reference counterpart beyond the inlet/outlet
  * flags of kernel.cu:1081-1088): x = 0 inlet, x = nx-1 outlet, the four side
- * faces are fixed walls and override inlet/outlet. */
+ * faces are fixed walls and override inlet/outlet.  The pipe uses the duct's
+ * types under its solid mask (applied by the caller). */
 static int node_type(const orc_params* p, int x, int y, int z) {
   int t = ORC_NODE_FLUID;
   if (p->casetype == ORC_CAVITY) {
@@ -465,9 +493,13 @@ void orc_initialize(orc_lattice* L) {
             ds = in ? (ds | 1) << 1 : ds << 1;
           }
           L->dirstate[n] = ds | 1;
-          const int ty = node_type(p, x, y, z);
+          const int ty = (L->mask && L->mask[n]) ? ORC_NODE_STRUCT : node_type(p, x, y, z);
           L->type[n] = ty;
-          if (ty == ORC_NODE_STRUCT) continue;
+          if (ty == ORC_NODE_STRUCT) { /* keInitialize skips it (kernel.cu:294): zeros */
+            memset(&L->arF[n * ORC_Q], 0, ORC_Q * sizeof(real));
+            L->rho[n] = L->ux[n] = L->uy[n] = L->uz[n] = L->prev_rho[n] = L->tau_eddy[n] = 0;
+            continue;
+          }
           real u[3] = {0, 0, 0};
           if (ty == ORC_NODE_MOVINGWALL && p->init_lid_velocity) {
             u[0] = (real)p->ref_vel[0]; u[1] = (real)p->ref_vel[1]; u[2] = (real)p->ref_vel[2];
@@ -516,7 +548,12 @@ static void phase_collide(orc_lattice* L) {
 }
 
 /* Streaming vertex (lbm_codelets.cpp:37-104): pull through the tile's padded
- * arFTemp slice with the tile's Move offsets (runSimulation.cpp:101). */
+ * arFTemp slice with the tile's Move offsets (runSimulation.cpp:101).  A solid
+ * donor sends nothing (keStreaming, kernel.cu:494, 511); the link bounces back
+ * at the node instead, f_opp(q)(x) = f*_q(x) (keBoundaryStruct, kernel.cu:
+ * 659-697).  The reference applies that rule to FLUID nodes only and leaves
+ * such links of inlet/outlet/wall nodes holding the previous step's value;
+ * here every non-solid node bounces back. */
 static void phase_stream(orc_lattice* L) {
   for (int t = 0; t < L->ntile; ++t) {
     int c[10];
@@ -535,6 +572,10 @@ static void phase_stream(orc_lattice* L) {
             const int loc = x + z * nx2 + y * nslave2 + mv[q]; /* index inside the tile's padded slice */
             const int ly = loc / nslave2, rem = loc % nslave2, lz = rem / nx2, lx = rem % nx2;
             const size_t g = pid(L, c[0] + lx, c[2] + ly, c[4] + lz);
+            if (L->mask && L->mask[nid(L, c[0] + lx - 1, c[2] + ly - 1, c[4] + lz - 1)]) {
+              L->arF[n * ORC_Q + OPP[q]] = L->arFTemp[pid(L, x + c[0] + 1, y + c[2] + 1, z + c[4] + 1) * ORC_Q + q];
+              continue;
+            }
             L->arF[n * ORC_Q + OPP[q]] = L->arFTemp[g * ORC_Q + OPP[q]];
           }
         }

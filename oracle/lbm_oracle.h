@@ -41,10 +41,11 @@ extern "C" {
 
 #define ORC_CAVITY 1 /* common.h:26-27; 0 is the square-duct channel here */
 #define ORC_CHANNEL 0
+#define ORC_PIPE 2 /* common.h:27 POISEUILLE of the CUDA path: the duct with a circular solid mask */
 
 typedef struct orc_params {
   int nx, ny, nz;
-  int casetype;          /* ORC_CAVITY or ORC_CHANNEL */
+  int casetype;          /* ORC_CAVITY, ORC_CHANNEL or ORC_PIPE */
   int enable_mrt;        /* 1: MRT (Collision_MRT), 0: SRT (keCollision_SRT) */
   int enable_les;        /* Smagorinsky eddy viscosity */
   int init_lid_velocity; /* 1: lid nodes start at U_lid (lbm_codelets.cpp:215-220) */
@@ -55,6 +56,8 @@ typedef struct orc_params {
   unsigned long long seed;
   /* MRT relaxation rates; a negative entry means 1/(tau + tau_eddy). */
   double s_rates[ORC_Q];
+  /* pipe: node (x, y, z) is solid when (y - cy)^2 + (z - cz)^2 > radius^2 (kernel.cu:1090) */
+  double pipe_radius, pipe_cy, pipe_cz;
 } orc_params;
 
 typedef struct orc_lattice orc_lattice;
@@ -87,6 +90,9 @@ void orc_cell_srt(const orc_params* p, const double f[ORC_Q], double rho, const 
 /* ---- lattice ------------------------------------------------------------ */
 orc_lattice* orc_create(const orc_params* p, int ntile, const int slice[][3][2]);
 void orc_destroy(orc_lattice* L);
+/* Solid nodes (NODE_STRUCT), mask[ny][nz][nx] non-zero = solid; NULL restores the
+ * case's own geometry.  Takes effect at the next orc_initialize. */
+void orc_set_mask(orc_lattice* L, const unsigned char* mask);
 void orc_initialize(orc_lattice* L);
 void orc_step(orc_lattice* L);
 void orc_run(orc_lattice* L, int nsteps);

@@ -11,7 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 Q = 19
-CAVITY, CHANNEL = 1, 0
+CAVITY, CHANNEL, PIPE = 1, 0, 2
 NODE_FLUID, NODE_FIXEDWALL, NODE_MOVINGWALL = 1 << 1, 1 << 2, 1 << 3
 NODE_INLET, NODE_OUTLET, NODE_STRUCT = 1 << 4, 1 << 5, 1 << 6
 
@@ -27,6 +27,7 @@ class Params(ctypes.Structure):
         ("inlet_peak", ctypes.c_double), ("perturb", ctypes.c_double),
         ("seed", ctypes.c_ulonglong),
         ("s_rates", ctypes.c_double * Q),
+        ("pipe_radius", ctypes.c_double), ("pipe_cy", ctypes.c_double), ("pipe_cz", ctypes.c_double),
     ]
 
 
@@ -70,6 +71,7 @@ def lib(fp32=False):
         L.orc_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ip]
         L.orc_create.restype = vp
         L.orc_destroy.argtypes = [vp]
+        L.orc_set_mask.argtypes = [vp, vp]
         L.orc_initialize.argtypes = [vp]
         L.orc_step.argtypes = [vp]
         L.orc_run.argtypes = [vp, ctypes.c_int]
@@ -226,6 +228,15 @@ class Lattice:
 
     def initialize(self):
         self.L.orc_initialize(self.h)
+
+    def set_mask(self, mask):
+        """Solid nodes [ny][nz][nx] (None: the case's own geometry); before initialize()."""
+        if mask is None:
+            self.L.orc_set_mask(self.h, None)
+            return
+        m = np.ascontiguousarray(np.asarray(mask) != 0, dtype=np.uint8)
+        assert m.shape == self.shape
+        self.L.orc_set_mask(self.h, m.ctypes.data)
 
     def run(self, n):
         self.L.orc_run(self.h, n)

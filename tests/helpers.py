@@ -34,15 +34,19 @@ def oracle_params(c):
     return oracle.default_params(**c)
 
 
-def run_oracle(c, steps, slices=None, fp32=False):
+def run_oracle(c, steps, slices=None, fp32=False, mask=None):
     L = oracle.Lattice(oracle_params(c), slices, fp32=fp32)
+    if mask is not None:
+        L.set_mask(mask)
     L.initialize()
     L.run(steps)
     return L
 
 
-def run_gpu(c, steps, storage=nflow.F32, slices=None):
+def run_gpu(c, steps, storage=nflow.F32, slices=None, mask=None):
     s = nflow.Solver(gpu_input(c, storage), slices)
+    if mask is not None:
+        s.set_solid_mask(mask)
     s.initialize()
     if steps:
         s.run(steps)
@@ -63,12 +67,12 @@ def macro_errors(gpu_macro, orc_macro):
     return eu, er
 
 
-def fp32_parity(c, steps, gpu=None):
+def fp32_parity(c, steps, gpu=None, mask=None):
     """Run GPU (FP32), FP64 oracle and FP32 oracle; return a dict of errors
     and assert the GPU is within the round-off allowance."""
-    s = gpu if gpu is not None else run_gpu(c, steps)
-    o64 = run_oracle(c, steps)
-    o32 = run_oracle(c, steps, fp32=True)
+    s = gpu if gpu is not None else run_gpu(c, steps, mask=mask)
+    o64 = run_oracle(c, steps, mask=mask)
+    o32 = run_oracle(c, steps, fp32=True, mask=mask)
     m64 = o64.read_macro()
     eu, er = macro_errors(s.read_macro(), m64)
     eu32, er32 = macro_errors(o32.read_macro(), m64)
