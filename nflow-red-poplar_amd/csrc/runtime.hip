@@ -37,6 +37,11 @@ int validate_input(const nflow_input& in, int storage) {
   const unsigned long long n = (unsigned long long)in.nx * in.ny * in.nz;
   const unsigned long long pstride = round_up(n, 128) + NFLOW_PLANE_PAD;
   if (pstride * 19ull >= (1ull << 32)) return fail(NFLOW_EINVAL, "lattice of %llu nodes exceeds the 32-bit index range", n);
+  // the step kernel addresses a window of two x-z planes plus one work item
+  // with 32-bit byte offsets
+  const unsigned long long plane = (unsigned long long)in.nx * in.nz + in.nx;
+  if ((2 * plane + 1024) * 19ull * 4ull >= (1ull << 32))
+    return fail(NFLOW_EINVAL, "x-z plane of %d x %d nodes exceeds the 32-bit window range", in.nx, in.nz);
   if (!(in.tau > 0.5)) return fail(NFLOW_EINVAL, "tau must exceed 0.5 (got %g)", in.tau);
   if (!(in.ref_den > 0.0)) return fail(NFLOW_EINVAL, "ref_den must be positive");
   if (in.enable_les && in.smagorinsky < 0.0) return fail(NFLOW_EINVAL, "smagorinsky must be >= 0");
