@@ -13,6 +13,7 @@ channel | cavity256_f16 | cavity256_bf16.
 """
 import argparse
 import json
+import statistics
 import os
 import sys
 import time
@@ -61,8 +62,19 @@ def traffic_from_profiles(name):
     return d.get("bytes_per_launch")
 
 
-def cpu_baseline(w, budget_s=15.0):
-    """The FP64 oracle timed on this host on a bounded sample of the same workload."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, budget_s=15.0, single_budget_s=8.0):
+    """The FP64 oracle timed on this host on a bounded sample of the same
+    workload: all worker threads (the reported value) and one thread."""
     threads = min(os.cpu_count() or 1, 16)
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -74,18 +86,27 @@ def cpu_baseline(w, budget_s=15.0):
                               seed=2024)
     L = oracle.Lattice(p)
     L.initialize()
-    t = time.perf_counter()
-    L.run(1)
-    t1 = time.perf_counter() - t
-    steps = max(1, min(60, int(budget_s / max(t1, 1e-3))))
-    t = time.perf_counter()
-    L.run(steps)
-    dt = time.perf_counter() - t
     n = w["nx"] * w["ny"] * w["nz"]
-    return {"value": round(n * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": oracle.lib().orc_num_threads(),
-            "kind": "port",
+
+    def timed(budget, max_steps):
+        t = time.perf_counter()
+        L.run(1)
+        t1 = time.perf_counter() - t
+        steps = max(1, min(max_steps, int(budget / max(t1, 1e-3))))
+        t = time.perf_counter()
+        L.run(steps)
+        return steps, time.perf_counter() - t
+
+    cores = oracle.lib().orc_num_threads()
+    steps, dt = timed(budget_s, 60)
+    oracle.lib().orc_set_num_threads(1)
+    steps1, dt1 = timed(single_budget_s, 20)
+    oracle.lib().orc_set_num_threads(cores)
+    return {"value": round(n * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
             "sample": f"FP64 oracle (C, OpenMP), same workload, {steps} timed steps after 1 warm-up step "
-                      f"({dt:.1f} s)"}
+                      f"({dt:.1f} s) on {cores} threads; single thread {steps1} steps ({dt1:.1f} s)",
+            "single_thread_value": round(n * steps1 / dt1 / 1e6, 3), "cpu_model": cpu_model(),
+            "host_cpus": os.cpu_count()}
 
 
 def solo_run(nflow, inp, steps):
@@ -323,11 +344,19 @@ def main():
     b.initialize()
     b.run(args.warmup)
     b.synchronize()
+    # K timed steps in five segments, each timed with HIP events on the
+    # solver's stream; the kernel time reported is the median segment's
+    segs = [args.steps // 5 + (1 if i < args.steps % 5 else 0) for i in range(5)]
+    seg_ms = []
     t0 = time.perf_counter()
-    b.run(args.steps)
+    for k in segs:
+        if k == 0:
+            continue
+        b.run(k)
+        seg_ms.append(b.timing()[0])
     b.synchronize()
     wall = time.perf_counter() - t0
-    dev_ms, _ = b.timing()
+    dev_ms = statistics.median(seg_ms)
     m, mom, _ = b.diagnostics(0)
     n = w["nx"] * w["ny"] * w["nz"]
     mlups = n * args.steps / wall / 1e6
@@ -351,7 +380,8 @@ def main():
                    "tuning": b.get_tuning(), "device": name, "cus": cus},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(args.workload),
-                     "bytes_per_update": bpu, "kernel_ms": round(dev_ms, 5)},
+                     "bytes_per_update": bpu, "kernel_ms": round(dev_ms, 5),
+                     "kernel_ms_segments": [round(x, 5) for x in seg_ms]},
         "hbm_frac_of_peak": round(achieved / HBM_PEAK_GBS, 4),
         "copy_ceiling_gbs": round(copy_gbs, 1),
         "mass_drift": abs(m - n) / n,

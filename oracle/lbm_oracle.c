@@ -453,6 +453,14 @@ int orc_num_threads(void) {
 #endif
 }
 
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* Node type for global coordinates.  Cavity: lbm_codelets.cpp:183-201.
  * Channel (square duct, no reference counterpart beyond the inlet/outlet
  * flags of kernel.cu:1081-1088): x = 0 inlet, x = nx-1 outlet, the four side

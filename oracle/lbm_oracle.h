@@ -105,6 +105,7 @@ void orc_read_macro(const orc_lattice* L, double* rho, double* ux, double* uy, d
 void orc_read_node_type(const orc_lattice* L, int* type, int* dirstate);
 void orc_diagnostics(const orc_lattice* L, double* mass, double mom[3], double* drho_l1);
 int orc_num_threads(void);
+void orc_set_num_threads(int n); /* OpenMP worker count for later calls */
 /* sizeof the working precision: 8 for liboracle.so, 4 for liboracle_f32.so */
 int orc_precision_bytes(void);
 

@@ -83,6 +83,7 @@ def lib(fp32=False):
         L.orc_read_node_type.argtypes = [vp, ip, ip]
         L.orc_diagnostics.argtypes = [vp, dp, dp, dp]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
         L.orc_precision_bytes.restype = ctypes.c_int
         _libs[name] = L
     return _libs[name]
