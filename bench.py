@@ -248,6 +248,8 @@ def bench_serve(args):
         "completed_ok": ok, "wall_s": round(wall, 3), "node_updates": updates,
         "serial_plain_f32_mlups": round(serial_mlups, 1), "speedup_vs_serial_plain": round(mlups / serial_mlups, 3),
         "peak_resident": st["peak_resident"], "peak_bytes": st["peak_bytes"],
+        "server_busy_s": round(st["busy_s"], 3), "server_step_s": round(st["step_s"], 3), "ticks": st["ticks"],
+        "step_mlups": round(st["node_updates"] / max(st["step_s"], 1e-9) / 1e6, 1),
         "rel_l2_u_vs_fp32_solo": [round(e, 6) for e in errs],
         "latency_includes": "queueing: offered load exceeds capacity",
     }))

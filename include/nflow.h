@@ -218,6 +218,8 @@ typedef struct nflow_server_stats {
   double busy_s;            /* wall time with at least one resident lattice */
   size_t peak_bytes;        /* peak device bytes held by resident lattices */
   int peak_resident;
+  long long ticks;          /* step launches issued as one batch between host visits */
+  double step_s;            /* device time of those launches (HIP events around each tick) */
 } nflow_server_stats;
 
 int nflow_server_create(size_t hbm_budget_bytes, int storage, int max_resident, int steps_per_tick,

@@ -231,11 +231,16 @@ int Server::tick() {
   for (size_t i = 0; i < resident.size(); ++i)
     k = std::min(k, resident[i].target - batch.lats[i].steps_done);
   if (k > 0) {
-    int rc = batch.launch_steps((int)k);
+    int rc = batch.run((int)k);  // launch_steps between the batch's timing events
     if (rc) return rc;
     st.launches += k;
   }
   NFLOW_HIP(hipStreamSynchronize(batch.stream));
+  if (k > 0) {
+    double ms = 0.0;
+    if (batch.timing(&ms, nullptr) == 0) st.step_s += ms * (double)k * 1e-3;
+    st.ticks++;
+  }
   return 0;
 }
 

@@ -64,7 +64,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_longlong), ("admitted", ctypes.c_longlong),
                 ("completed", ctypes.c_longlong), ("rejected", ctypes.c_longlong),
                 ("node_updates", ctypes.c_longlong), ("launches", ctypes.c_longlong),
-                ("busy_s", ctypes.c_double), ("peak_bytes", ctypes.c_size_t), ("peak_resident", ctypes.c_int)]
+                ("busy_s", ctypes.c_double), ("peak_bytes", ctypes.c_size_t), ("peak_resident", ctypes.c_int),
+                ("ticks", ctypes.c_longlong), ("step_s", ctypes.c_double)]
 
 
 # every function declared in include/nflow.h with its ctypes signature
