@@ -11,17 +11,19 @@ pytestmark = pytest.mark.gpu
 
 
 def test_cavity_128_mrt_les_tau_051_against_oracle():
-    fp32_parity(cfg(nx=128, ny=128, nz=128, enable_les=1, smagorinsky=0.17, tau=0.51), 20)
+    # 200 steps: the lid's shear layer has spread over ~10 nodes, the LES
+    # viscosity is active along it (tau = 0.51 is close to the stability limit)
+    fp32_parity(cfg(nx=128, ny=128, nz=128, enable_les=1, smagorinsky=0.17, tau=0.51), 200)
 
 
 def test_cavity_256_mrt_les_against_oracle():
-    fp32_parity(cfg(nx=256, ny=256, nz=256, enable_les=1, smagorinsky=0.17, tau=0.6), 3)
+    fp32_parity(cfg(nx=256, ny=256, nz=256, enable_les=1, smagorinsky=0.17, tau=0.6), 20)
 
 
 def test_channel_512x128x128_against_oracle():
     c = cfg(nx=512, ny=128, nz=128, casetype=0, enable_les=1, smagorinsky=0.17, tau=0.56, inlet_peak=0.05,
             perturb=0.01, seed=2024)
-    fp32_parity(c, 12)
+    fp32_parity(c, 40)
 
 
 def test_cavity_256_conserves_mass_and_mirror_symmetry():
