@@ -331,6 +331,28 @@ __device__ inline float tau_eddy_les(const float f[Q], float rho, float ir, floa
   return 0.5f * (__builtin_sqrtf(tau * tau + les_c * __builtin_sqrtf(qq) * ir) - tau);
 }
 
+// The Smagorinsky relaxation time of tau_eddy_les from the MRT moments: the
+// second moments P = sum_q e e f are m13 = P_xy, m14 = P_yz, m15 = P_xz,
+// m9 = 3 P_xx - tr P, m11 = P_yy - P_zz and m1 = 19 tr P - 30 rho, and
+// sum_q e e feq = rho/3 I + j j / rho, so Pi = P - rho/3 I - j j / rho.
+__device__ inline float les_from_moments(const float m[Q], float rho, float ir, float jx, float jy, float jz,
+                                         float tau, float les_c) {
+  const float tr = (m[1] + 30.0f * rho) * (1.0f / 19.0f);
+  const float pxx = (m[9] + tr) * (1.0f / 3.0f);
+  const float ryz = tr - pxx;
+  const float pyy = 0.5f * (ryz + m[11]);
+  const float pzz = 0.5f * (ryz - m[11]);
+  const float r3 = rho * (1.0f / 3.0f);
+  const float qxx = pxx - r3 - jx * jx * ir;
+  const float qyy = pyy - r3 - jy * jy * ir;
+  const float qzz = pzz - r3 - jz * jz * ir;
+  const float qxy = m[13] - jx * jy * ir;
+  const float qyz = m[14] - jy * jz * ir;
+  const float qxz = m[15] - jx * jz * ir;
+  const float qq = qxx * qxx + qyy * qyy + qzz * qzz + 2.0f * (qxy * qxy + qyz * qyz + qxz * qxz);
+  return 0.5f * (__builtin_sqrtf(tau * tau + les_c * __builtin_sqrtf(qq) * ir) - tau);
+}
+
 // BGK, kernel.cu:479-486.
 __device__ inline void collide_srt(float f[Q], float rho, float ir, float jx, float jy, float jz, float omega) {
   const float ux = jx * ir, uy = jy * ir, uz = jz * ir;
@@ -352,8 +374,12 @@ __device__ inline void collide_srt(float f[Q], float rho, float ir, float jx, fl
 
 // MRT, lbm_codelets.cpp:449-537: f* = f - M^-1 S (M f - meq), with the even /
 // odd split of the orthogonal basis (each row touches only f_a + f_opp(a) or
-// only f_a - f_opp(a)) and M^-1 = M^T diag(1/|row|^2).
-__device__ inline void collide_mrt(float f[Q], float rho, float jx, float jy, float jz, float snu, const LatticeDev& L) {
+// only f_a - f_opp(a)) and M^-1 = M^T diag(1/|row|^2).  With LES the
+// eddy relaxation time comes from the same moments (les_from_moments).
+__device__ inline float les_from_moments(const float m[Q], float rho, float ir, float jx, float jy, float jz,
+                                         float tau, float les_c);
+__device__ inline void collide_mrt(float f[Q], float rho, float ir, float jx, float jy, float jz,
+                                   const LatticeDev& L) {
   float s[NPAIR], d[NPAIR];
 #pragma unroll
   for (int i = 0; i < NPAIR; ++i) {
@@ -386,6 +412,7 @@ __device__ inline void collide_mrt(float f[Q], float rho, float jx, float jy, fl
     }
     m[k] = acc;
   }
+  const float snu = 1.0f / (L.tau + (L.les ? les_from_moments(m, rho, ir, jx, jy, jz, L.tau, L.les_c) : 0.0f));
   const float ir0 = L.inv_refden;
   const float jxx = jx * jx, jyy = jy * jy, jzz = jz * jz, jj = jxx + jyy + jzz;
   float meq[Q];
@@ -482,11 +509,10 @@ __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], const Ac
       so[(SIDE_U_OUT1 + 2) * pl + yz] = jz * ir;
     }
   }
-  float te = 0.0f;
-  if (L.les) te = tau_eddy_les(f, rho, ir, jx, jy, jz, L.tau, L.les_c);
   if (L.model == MODEL_MRT) {
-    collide_mrt(f, rho, jx, jy, jz, 1.0f / (L.tau + te), L);
+    collide_mrt(f, rho, ir, jx, jy, jz, L);
   } else {
+    const float te = L.les ? tau_eddy_les(f, rho, ir, jx, jy, jz, L.tau, L.les_c) : 0.0f;
     collide_srt(f, rho, ir, jx, jy, jz, 1.0f / (L.tau + te));
   }
   const uint32_t ps = (uint32_t)L.pstride;
