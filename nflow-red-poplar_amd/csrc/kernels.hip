@@ -288,6 +288,13 @@ __device__ inline bool is_solid(const LatticeDev& L, uint32_t n) {
 // ---------------------------------------------------------------------------
 // Collision
 // ---------------------------------------------------------------------------
+// Hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32, each within
+// 1 ulp) for the per-node 1/rho, 1/(tau + tau_eddy) and the two LES roots:
+// the correctly rounded forms expand to ~10 instructions each on the
+// critical path of the collision.
+__device__ inline float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ inline void moments(const float f[Q], float& rho, float& jx, float& jy, float& jz) {
   float r = f[0];
 #pragma unroll
@@ -328,7 +335,7 @@ __device__ inline float tau_eddy_les(const float f[Q], float rho, float ir, floa
   pyz -= jy * jz * ir;
   pxz -= jx * jz * ir;
   const float qq = pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pyz * pyz + pxz * pxz);
-  return 0.5f * (__builtin_sqrtf(tau * tau + les_c * __builtin_sqrtf(qq) * ir) - tau);
+  return 0.5f * (hw_sqrt(tau * tau + les_c * hw_sqrt(qq) * ir) - tau);
 }
 
 // The Smagorinsky relaxation time of tau_eddy_les from the MRT moments: the
@@ -350,7 +357,7 @@ __device__ inline float les_from_moments(const float m[Q], float rho, float ir, 
   const float qyz = m[14] - jy * jz * ir;
   const float qxz = m[15] - jx * jz * ir;
   const float qq = qxx * qxx + qyy * qyy + qzz * qzz + 2.0f * (qxy * qxy + qyz * qyz + qxz * qxz);
-  return 0.5f * (__builtin_sqrtf(tau * tau + les_c * __builtin_sqrtf(qq) * ir) - tau);
+  return 0.5f * (hw_sqrt(tau * tau + les_c * hw_sqrt(qq) * ir) - tau);
 }
 
 // BGK, kernel.cu:479-486.
@@ -412,7 +419,7 @@ __device__ inline void collide_mrt(float f[Q], float rho, float ir, float jx, fl
     }
     m[k] = acc;
   }
-  const float snu = 1.0f / (L.tau + (L.les ? les_from_moments(m, rho, ir, jx, jy, jz, L.tau, L.les_c) : 0.0f));
+  const float snu = hw_rcp(L.tau + (L.les ? les_from_moments(m, rho, ir, jx, jy, jz, L.tau, L.les_c) : 0.0f));
   const float ir0 = L.inv_refden;
   const float jxx = jx * jx, jyy = jy * jy, jzz = jz * jz, jj = jxx + jyy + jzz;
   float meq[Q];
@@ -493,7 +500,7 @@ __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], const Ac
                                        uint32_t nr, int x, int y, int z) {
   float rho, jx, jy, jz;
   moments(f, rho, jx, jy, jz);
-  const float ir = 1.0f / rho;
+  const float ir = hw_rcp(rho);
   if (L.casetype != CASE_CAVITY) {  // channel and pipe: inlet / outlet side planes
     const int pl = L.ny * L.nz, yz = y * L.nz + z;
     const gptr<float> so = (gptr<float>)side_out;
@@ -513,7 +520,7 @@ __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], const Ac
     collide_mrt(f, rho, ir, jx, jy, jz, L);
   } else {
     const float te = L.les ? tau_eddy_les(f, rho, ir, jx, jy, jz, L.tau, L.les_c) : 0.0f;
-    collide_srt(f, rho, ir, jx, jy, jz, 1.0f / (L.tau + te));
+    collide_srt(f, rho, ir, jx, jy, jz, hw_rcp(L.tau + te));
   }
   const uint32_t ps = (uint32_t)L.pstride;
 #pragma unroll
