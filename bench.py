@@ -321,7 +321,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab decomposition is "
-                                   "a design (DESIGN.md), not code", "n_gpus": args.gpus}))
+                                   "designed but not built", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.mode == "batch":
         return bench_batch(args)

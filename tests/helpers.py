@@ -5,7 +5,7 @@ import numpy as np
 import nflow
 import oracle
 
-# FP32 parity is judged against the FP64 oracle, with the allowance set by
+# FP32 parity is measured against the FP64 oracle, with the allowance set by
 # round-off itself: the oracle's own algorithm evaluated in float
 # (liboracle_f32.so) drifts from FP64 by err32; the GPU, which evaluates the
 # same maths in float in another order (fused multiply-adds, the orthogonal

@@ -40,7 +40,7 @@ def test_cavity_matches_oracle(model, les, tau):
 
 
 def test_cavity_32_srt_1000_steps():
-    """32^3 SRT cavity, tau 0.6, lid 0.05, rest start, 1000 steps."""
+    """The CPU-plain case: 32^3 SRT cavity, tau 0.6, lid 0.05, rest start, 1000 steps."""
     c = cfg(nx=32, ny=32, nz=32, enable_mrt=0, enable_les=0, tau=0.6)
     fp32_parity(c, 1000)
 
