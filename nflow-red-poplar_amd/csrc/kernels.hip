@@ -193,7 +193,8 @@ __device__ inline int node_type(const LatticeDev& L, int x, int y, int z) {
     if (y == 0) t = NODE_FIXEDWALL;
     if (y == L.ny - 1) t = NODE_MOVINGWALL;
     if (x == 0 || x == L.nx - 1 || z == 0 || z == L.nz - 1) t = NODE_FIXEDWALL;
-  } else {  // square duct: inlet x = 0, outlet x = nx-1, side faces are walls
+  } else {  // channel and pipe (a duct; the pipe adds its solid mask): inlet x = 0,
+            // outlet x = nx-1, side faces are walls (kernel.cu:1081-1088 for the flags)
     if (x == 0) t = NODE_INLET;
     if (x == L.nx - 1) t = NODE_OUTLET;
     if (y == 0 || y == L.ny - 1 || z == 0 || z == L.nz - 1) t = NODE_FIXEDWALL;

@@ -233,12 +233,14 @@ int Server::tick() {
   if (k > 0) {
     int rc = batch.run((int)k);  // launch_steps between the batch's timing events
     if (rc) return rc;
-    st.launches += k;
   }
   NFLOW_HIP(hipStreamSynchronize(batch.stream));
   if (k > 0) {
     double ms = 0.0;
-    if (batch.timing(&ms, nullptr) == 0) st.step_s += ms * (double)k * 1e-3;
+    const bool timed = batch.timing(&ms, nullptr) == 0;
+    std::lock_guard<std::mutex> lk(mu);  // stats are read by nflow_server_stats_get
+    st.launches += k;
+    if (timed) st.step_s += ms * (double)k * 1e-3;
     st.ticks++;
   }
   return 0;
@@ -299,7 +301,10 @@ void Server::loop() {
     for (size_t i = resident.size(); i-- > 0;)
       if (batch.lats[i].steps_done >= resident[i].target) finish(i, t2);
   }
-  if (busy_from >= 0.0) st.busy_s += now() - busy_from;
+  if (busy_from >= 0.0) {
+    std::lock_guard<std::mutex> lk(mu);
+    st.busy_s += now() - busy_from;
+  }
 }
 
 }  // namespace nflow
