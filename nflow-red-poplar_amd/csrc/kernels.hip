@@ -77,6 +77,8 @@ __device__ __forceinline__ uint32_t pidx(int q, uint32_t n, uint32_t ps) {
 // accesses, which it must otherwise use for pointers read out of a struct.
 template <class T>
 using gptr = __attribute__((address_space(1))) T*;
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // Population access.  GPops: a 64-bit pointer per lane (global_load/store).
@@ -236,26 +238,52 @@ __device__ inline void pull(const LatticeDev& L, const Acc& in, uint32_t n, uint
   }
 }
 
+// Inputs of the inlet / outlet rules from the previous step's side planes:
+// inlet, v[0] = rho(x+1) and v[1] = u_in; outlet, v[0..2] = u(nx-1) and
+// v[3..5] = u(nx-2).  Loaded before the 19 pulls, so that a wave holding an
+// inlet or outlet lane waits for both together rather than in sequence.
+struct SideIn {
+  float v[6];
+};
+
+__device__ inline SideIn side_inputs(const LatticeDev& L, const float* __restrict__ side, int x, int y, int z) {
+  SideIn si = {{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}};
+  if (L.casetype != CASE_CAVITY && (x == 0 || x == L.nx - 1) &&
+      !(y == 0 || y == L.ny - 1 || z == 0 || z == L.nz - 1)) {  // side faces are walls (node_type)
+    const gptr<const float> rec = (gptr<const float>)side + (size_t)(y * L.nz + z) * SIDE_REC;
+    if (x == 0) {
+      const f32x2 a = *(gptr<const f32x2>)(rec + SIDE_RHO);  // rho(x = 1), u_in
+      si.v[0] = a.x;
+      si.v[1] = a.y;
+    } else {
+      const f32x4 u0 = *(gptr<const f32x4>)(rec + SIDE_U0);
+      const f32x4 u1 = *(gptr<const f32x4>)(rec + SIDE_U1);
+      si.v[0] = u0.x;
+      si.v[1] = u0.y;
+      si.v[2] = u0.z;
+      si.v[3] = u1.x;
+      si.v[4] = u1.y;
+      si.v[5] = u1.z;
+    }
+  }
+  return si;
+}
+
 // Wall / inlet / outlet rules on the links pull() filled with f*_opp(q)(x).
-__device__ inline void boundary(const LatticeDev& L, const float* __restrict__ side, int x, int y, int z, float f[Q]) {
+__device__ inline void boundary(const LatticeDev& L, const SideIn& si, int x, int y, int z, float f[Q]) {
   const bool xlo = x == 0, xhi = x == L.nx - 1;
   const bool ylo = y == 0, yhi = y == L.ny - 1;
   const bool zlo = z == 0, zhi = z == L.nz - 1;
   if (xlo | xhi | ylo | yhi | zlo | zhi) {
     const int t = node_type(L, x, y, z);
     if (t == NODE_FIXEDWALL) return;  // plain bounce-back: already in place
-    const gptr<const float> sd = (gptr<const float>)side;
     float a = 0.0f, ubx = 0.0f, uby = 0.0f, ubz = 0.0f, usq = 0.0f;
     if (t == NODE_INLET) {
-      const int yz = y * L.nz + z;
-      a = 6.0f * sd[SIDE_RHO_IN * L.ny * L.nz + yz] * ((gptr<const float>)L.uin)[yz];  // rho(x+1), previous step
+      a = 6.0f * si.v[0] * si.v[1];  // 6 rho(x+1) u_in, previous step
     } else if (t == NODE_OUTLET) {
-      const int yz = y * L.nz + z, pl = L.ny * L.nz;
-      const gptr<const float> u0 = sd + SIDE_U_OUT * pl + yz;
-      const gptr<const float> u1 = sd + SIDE_U_OUT1 * pl + yz;
-      ubx = u0[0] + 0.5f * (u0[0] - u1[0]);
-      uby = u0[pl] + 0.5f * (u0[pl] - u1[pl]);
-      ubz = u0[2 * pl] + 0.5f * (u0[2 * pl] - u1[2 * pl]);
+      ubx = si.v[0] + 0.5f * (si.v[0] - si.v[3]);
+      uby = si.v[1] + 0.5f * (si.v[1] - si.v[4]);
+      ubz = si.v[2] + 0.5f * (si.v[2] - si.v[5]);
       usq = ubx * ubx + uby * uby + ubz * ubz;
     }
 #pragma unroll
@@ -278,8 +306,9 @@ __device__ inline void boundary(const LatticeDev& L, const float* __restrict__ s
 template <class St, bool MASK, class Acc>
 __device__ inline void gather(const LatticeDev& L, const Acc& in, const float* __restrict__ side, uint32_t n,
                               uint32_t nr, int x, int y, int z, float f[Q]) {
+  const SideIn si = side_inputs(L, side, x, y, z);
   pull<St, MASK>(L, in, n, nr, x, y, z, f);
-  boundary(L, side, x, y, z, f);
+  boundary(L, si, x, y, z, f);
 }
 
 __device__ inline bool is_solid(const LatticeDev& L, uint32_t n) {
@@ -502,20 +531,11 @@ __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], const Ac
   float rho, jx, jy, jz;
   moments(f, rho, jx, jy, jz);
   const float ir = hw_rcp(rho);
-  if (L.casetype != CASE_CAVITY) {  // channel and pipe: inlet / outlet side planes
-    const int pl = L.ny * L.nz, yz = y * L.nz + z;
-    const gptr<float> so = (gptr<float>)side_out;
-    if (x == 1) so[SIDE_RHO_IN * pl + yz] = rho;
-    if (x == L.nx - 1) {
-      so[(SIDE_U_OUT + 0) * pl + yz] = jx * ir;
-      so[(SIDE_U_OUT + 1) * pl + yz] = jy * ir;
-      so[(SIDE_U_OUT + 2) * pl + yz] = jz * ir;
-    }
-    if (x == L.nx - 2) {
-      so[(SIDE_U_OUT1 + 0) * pl + yz] = jx * ir;
-      so[(SIDE_U_OUT1 + 1) * pl + yz] = jy * ir;
-      so[(SIDE_U_OUT1 + 2) * pl + yz] = jz * ir;
-    }
+  if (L.casetype != CASE_CAVITY) {  // channel and pipe: inlet / outlet side records
+    const gptr<float> rec = (gptr<float>)side_out + (size_t)(y * L.nz + z) * SIDE_REC;
+    if (x == 1) rec[SIDE_RHO] = rho;
+    if (x == L.nx - 1) *(gptr<f32x4>)(rec + SIDE_U0) = f32x4{jx * ir, jy * ir, jz * ir, 0.0f};
+    if (x == L.nx - 2) *(gptr<f32x4>)(rec + SIDE_U1) = f32x4{jx * ir, jy * ir, jz * ir, 0.0f};
   }
   if (L.model == MODEL_MRT) {
     collide_mrt(f, rho, ir, jx, jy, jz, L);
@@ -613,13 +633,11 @@ __global__ void __launch_bounds__(256) k_init(LatticeDev L, int cur, const float
       b1.put(pidx<typename St::T>(q, n, ps), St::enc(-shift_of<St>(q)));
     }
     if (L.casetype != CASE_CAVITY && (x == 1 || x >= L.nx - 2)) {
-      const int pl = L.ny * L.nz, yz = y * L.nz + z;
       for (int b = 0; b < 2; ++b) {
-        const gptr<float> so = (gptr<float>)L.side[b];
-        if (x == 1) so[SIDE_RHO_IN * pl + yz] = 0.0f;
-        const int base = x == L.nx - 1 ? SIDE_U_OUT : SIDE_U_OUT1;
-        if (x >= L.nx - 2)
-          for (int c = 0; c < 3; ++c) so[(base + c) * pl + yz] = 0.0f;
+        const gptr<float> rec = (gptr<float>)L.side[b] + (size_t)(y * L.nz + z) * SIDE_REC;
+        if (x == 1) rec[SIDE_RHO] = 0.0f;
+        if (x == L.nx - 1) *(gptr<f32x4>)(rec + SIDE_U0) = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        if (x == L.nx - 2) *(gptr<f32x4>)(rec + SIDE_U1) = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       }
     }
     return;

@@ -18,10 +18,13 @@ enum : int {
   NODE_STRUCT = 1 << 6
 };
 
-// Channel boundary planes kept between steps, each [ny][nz]: the density of
-// the x = 1 plane (used by the inlet, kernel.cu:596-597) and the velocity of
-// the x = nx-1 and x = nx-2 planes (used by the outlet, kernel.cu:645-648).
-enum : int { SIDE_RHO_IN = 0, SIDE_U_OUT = 1, SIDE_U_OUT1 = 4, SIDE_PLANES = 7 };
+// Inlet / outlet state kept between steps (channel and pipe), one record of
+// SIDE_REC floats per (y, z), [ny][nz]: the density of the x = 1 node (used by
+// the inlet, kernel.cu:596-597) next to the inlet velocity u_in, and the
+// velocities of the x = nx-1 and x = nx-2 nodes (used by the outlet,
+// kernel.cu:645-648), each in its own 16-byte slot, so that an inlet node
+// reads its inputs with one load and an outlet node with two.
+enum : int { SIDE_RHO = 0, SIDE_UIN = 1, SIDE_U0 = 4, SIDE_U1 = 8, SIDE_REC = 12 };
 
 // Exact unsigned division by a runtime constant d for n < 2^31:
 // floor(n / d) = (n * magic) >> shift with shift = 31 + ceil(log2 d) and
@@ -46,13 +49,13 @@ inline FastDiv make_fastdiv(uint32_t d) {
   return r;
 }
 
-// One lattice as the kernels see it.  Populations are SoA planes
-// buf[i][q * pstride + n] with n = x + nx*(z + nz*y); the two buffers
-// alternate each step (buffer parity ^ parity0 is the one read).
+// One lattice as the kernels see it.  Populations of node n = x + nx*(z + nz*y)
+// live in buf[i] at the element pidx(q, n) of kernels.hip (256-byte runs of
+// consecutive nodes per population); the two buffers and the two side-record
+// arrays alternate each step (parity ^ parity0 is the one read).
 struct alignas(16) LatticeDev {
   void* buf[2];
   float* side[2];
-  const float* uin;   // inlet velocity profile [ny][nz] (channel, pipe)
   const uint8_t* mask;  // solid nodes (non-zero), or null
   long long pstride;  // elements between population planes
   int nx, ny, nz, nxz;

@@ -49,7 +49,7 @@ int validate_input(const nflow_input& in, int storage) {
 }
 
 struct Layout {
-  size_t pstride, pop, side, uin, mask, total;
+  size_t pstride, pop, side, mask, total;
 };
 
 static Layout layout_of(const nflow_input& in, int storage) {
@@ -57,10 +57,9 @@ static Layout layout_of(const nflow_input& in, int storage) {
   const size_t n = (size_t)in.nx * in.ny * in.nz;
   l.pstride = round_up(n, 128) + NFLOW_PLANE_PAD;
   l.pop = round_up(19 * l.pstride * elem_size(storage), 256);
-  l.side = round_up((size_t)SIDE_PLANES * in.ny * in.nz * sizeof(float), 256);
-  l.uin = round_up((size_t)in.ny * in.nz * sizeof(float), 256);
+  l.side = round_up((size_t)SIDE_REC * in.ny * in.nz * sizeof(float), 256);
   l.mask = in.casetype == NFLOW_PIPE ? round_up(n, 256) : 0;  // one byte per node
-  l.total = 2 * l.pop + 2 * l.side + l.uin + l.mask;
+  l.total = 2 * l.pop + 2 * l.side + l.mask;
   return l;
 }
 
@@ -84,8 +83,7 @@ void lattice_layout(HostLattice& h, const nflow_input& in, int storage, void* me
   d.buf[1] = p + l.pop;
   d.side[0] = reinterpret_cast<float*>(p + 2 * l.pop);
   d.side[1] = reinterpret_cast<float*>(p + 2 * l.pop + l.side);
-  d.uin = reinterpret_cast<const float*>(p + 2 * l.pop + 2 * l.side);
-  d.mask = l.mask ? reinterpret_cast<const uint8_t*>(p + 2 * l.pop + 2 * l.side + l.uin) : nullptr;
+  d.mask = l.mask ? reinterpret_cast<const uint8_t*>(p + 2 * l.pop + 2 * l.side) : nullptr;
   d.pstride = (long long)l.pstride;
   d.nx = in.nx;
   d.ny = in.ny;
@@ -124,20 +122,23 @@ static uint64_t splitmix64(uint64_t x) {
 int lattice_upload_inlet(HostLattice& h, hipStream_t st) {
   const nflow_input& in = h.in;
   if (in.casetype == NFLOW_CAVITY) return 0;
-  std::vector<float> u((size_t)in.ny * in.nz);
+  // side records of both step parities: u_in in its slot, the rest zero until
+  // the kernels write them
+  std::vector<float> rec((size_t)in.ny * in.nz * SIDE_REC, 0.0f);
   for (int y = 0; y < in.ny; ++y)
     for (int z = 0; z < in.nz; ++z) {
+      float& u = rec[((size_t)y * in.nz + z) * SIDE_REC + SIDE_UIN];
       if (in.casetype == NFLOW_PIPE) {  // keBoundaryInlet, kernel.cu:599-608: uniform RefVelx
-        u[(size_t)y * in.nz + z] = (float)in.ref_vel[0];
+        u = (float)in.ref_vel[0];
         continue;
       }
       const double eta = (y + 0.5) / in.ny, zeta = (z + 0.5) / in.nz;
       const uint64_t key = (uint64_t)in.seed ^ (((uint64_t)(uint32_t)y << 32) | (uint64_t)(uint32_t)z);
       const double xi = 2.0 * ((double)(splitmix64(key) >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
-      u[(size_t)y * in.nz + z] =
-          (float)(in.inlet_peak * 16.0 * eta * (1.0 - eta) * zeta * (1.0 - zeta) * (1.0 + in.perturb * xi));
+      u = (float)(in.inlet_peak * 16.0 * eta * (1.0 - eta) * zeta * (1.0 - zeta) * (1.0 + in.perturb * xi));
     }
-  NFLOW_HIP(hipMemcpyAsync(const_cast<float*>(h.dev.uin), u.data(), u.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  for (int b = 0; b < 2; ++b)
+    NFLOW_HIP(hipMemcpyAsync(h.dev.side[b], rec.data(), rec.size() * sizeof(float), hipMemcpyHostToDevice, st));
   std::vector<uint8_t> m;
   if (in.casetype == NFLOW_PIPE) {  // keSetNodeType, kernel.cu:1078-1094
     m.resize((size_t)h.dev.n);
