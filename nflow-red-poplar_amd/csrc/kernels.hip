@@ -270,36 +270,37 @@ __device__ inline SideIn side_inputs(const LatticeDev& L, const float* __restric
 }
 
 // Wall / inlet / outlet rules on the links pull() filled with f*_opp(q)(x).
+// node_type gives the x, z and y = 0 faces priority as fixed walls, so a
+// moving-wall node lies on y = ny-1 (and on y = 0 only when ny = 1), an inlet
+// node on x = 0 alone and an outlet node on x = nx-1 alone (nx >= 3): each
+// rule touches a fixed set of directions.
 __device__ inline void boundary(const LatticeDev& L, const SideIn& si, int x, int y, int z, float f[Q]) {
   const bool xlo = x == 0, xhi = x == L.nx - 1;
   const bool ylo = y == 0, yhi = y == L.ny - 1;
   const bool zlo = z == 0, zhi = z == L.nz - 1;
   if (xlo | xhi | ylo | yhi | zlo | zhi) {
     const int t = node_type(L, x, y, z);
-    if (t == NODE_FIXEDWALL) return;  // plain bounce-back: already in place
-    float a = 0.0f, ubx = 0.0f, uby = 0.0f, ubz = 0.0f, usq = 0.0f;
-    if (t == NODE_INLET) {
-      a = 6.0f * si.v[0] * si.v[1];  // 6 rho(x+1) u_in, previous step
-    } else if (t == NODE_OUTLET) {
-      ubx = si.v[0] + 0.5f * (si.v[0] - si.v[3]);
-      uby = si.v[1] + 0.5f * (si.v[1] - si.v[4]);
-      ubz = si.v[2] + 0.5f * (si.v[2] - si.v[5]);
-      usq = ubx * ubx + uby * uby + ubz * ubz;
-    }
+    if (t == NODE_MOVINGWALL) {  // f_q = f*_opp + 6 w rho0 (e_q . U), links leaving through y = ny-1
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const bool out = (EX[q] > 0 && xlo) || (EX[q] < 0 && xhi) || (EY[q] > 0 && ylo) || (EY[q] < 0 && yhi) ||
-                       (EZ[q] > 0 && zlo) || (EZ[q] < 0 && zhi);
-      if (!out) continue;
-      if (t == NODE_MOVINGWALL) {  // f_q = f*_opp + 6 w rho0 (e_q . U)
-        f[q] += 6.0f * W[q] * L.refden * edot(q, L.ulid[0], L.ulid[1], L.ulid[2]);
-      } else if (t == NODE_INLET) {  // f_q = f*_opp + 6 w rho(x+1) (e_q . u_in)
-        if (EX[q] != 0) f[q] += (EX[q] > 0 ? W[q] : -W[q]) * a;
-      } else if (t == NODE_OUTLET) {  // anti-bounce-back at rho0
-        const float eu = edot(q, ubx, uby, ubz);
-        f[q] = -f[q] + 2.0f * W[q] * L.refden * (1.0f + 4.5f * eu * eu - 1.5f * usq);
-      }
-    }
+      for (int q = 1; q < Q; ++q)
+        if (EY[q] < 0 || (EY[q] > 0 && ylo)) f[q] += 6.0f * W[q] * L.refden * edot(q, L.ulid[0], L.ulid[1], L.ulid[2]);
+    } else if (t == NODE_INLET) {  // f_q = f*_opp + 6 w rho(x+1) (e_q . u_in), e_x = +1
+      const float a = 6.0f * si.v[0] * si.v[1];  // rho(x = 1) and u_in of the previous step
+#pragma unroll
+      for (int q = 1; q < Q; ++q)
+        if (EX[q] > 0) f[q] += W[q] * a;
+    } else if (t == NODE_OUTLET) {  // anti-bounce-back at rho0, e_x = -1
+      const float ubx = si.v[0] + 0.5f * (si.v[0] - si.v[3]);
+      const float uby = si.v[1] + 0.5f * (si.v[1] - si.v[4]);
+      const float ubz = si.v[2] + 0.5f * (si.v[2] - si.v[5]);
+      const float usq = ubx * ubx + uby * uby + ubz * ubz;
+#pragma unroll
+      for (int q = 1; q < Q; ++q)
+        if (EX[q] < 0) {
+          const float eu = edot(q, ubx, uby, ubz);
+          f[q] = -f[q] + 2.0f * W[q] * L.refden * (1.0f + 4.5f * eu * eu - 1.5f * usq);
+        }
+    }  // fixed walls: plain bounce-back, already in place
   }
 }
 
