@@ -368,6 +368,7 @@ def main():
     # side by side (the population streams of a plane-per-population layout)
     copy1_gbs, _ = nflow.copy_bandwidth(2 * 19 * 4 * 4_000_000, streams=1, reps=20)
     copy_gbs, _ = nflow.copy_bandwidth(2 * 19 * 4 * 4_000_000, streams=19, reps=20)
+    copy_runs_gbs, _ = nflow.copy_bandwidth(2 * 19 * 4 * 4_000_000, streams=0, reps=20)
     out = {
         "metric": METRIC,
         "value": round(mlups, 1),
@@ -388,7 +389,8 @@ def main():
                      "bytes_per_update": bpu, "kernel_ms": round(dev_ms, 5),
                      "kernel_ms_segments": [round(x, 5) for x in seg_ms]},
         "hbm_frac_of_peak": round(achieved / HBM_PEAK_GBS, 4),
-        "copy_ceiling_gbs": {"1_stream": round(copy1_gbs, 1), "19_streams": round(copy_gbs, 1)},
+        "copy_ceiling_gbs": {"1_stream": round(copy1_gbs, 1), "19_streams": round(copy_gbs, 1),
+                             "population_runs": round(copy_runs_gbs, 1)},
         "mass_drift": abs(m - n) / n,
     }
     b.close()

@@ -233,7 +233,9 @@ void nflow_server_destroy(nflow_server* s);
 
 /* ---- measurement helpers ---------------------------------------------------
  * Device copy of `bytes` bytes split over `streams` equal arrays (a streaming
- * ceiling for the population traffic), timed with HIP events. */
+ * ceiling for the population traffic), timed with HIP events.  streams = 0
+ * copies in the FP32 population layout instead: blocks of 19 runs of 64
+ * floats, one 4-byte element of every run per thread. */
 int nflow_copy_bandwidth(size_t bytes, int streams, int reps, double* gbps, double* ms_per_copy);
 int nflow_device_info(char* name, int name_len, int* cu_count, size_t* hbm_bytes);
 

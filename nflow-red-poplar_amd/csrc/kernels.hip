@@ -807,6 +807,21 @@ __global__ void __launch_bounds__(256) k_copy_streams(const float4* __restrict__
   for (int s = 0; s < streams; ++s) dst[(size_t)s * per4 + i] = src[(size_t)s * per4 + i];
 }
 
+// The same streaming ceiling in the FP32 population layout itself: every
+// thread moves one 4-byte element of each of the 19 runs of its 64-node block,
+// as k_step's own-node stores do, with the same non-temporal hints.
+__global__ void __launch_bounds__(256) k_copy_runs(const float* __restrict__ src, float* __restrict__ dst,
+                                                   size_t nblk) {
+  const size_t node = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const size_t blk = node / 64, lane = node % 64;
+  if (blk >= nblk) return;
+  float v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = __builtin_nontemporal_load((gptr<const float>)src + (blk * Q + q) * 64 + lane);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(v[q], (gptr<float>)dst + (blk * Q + q) * 64 + lane);
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -892,6 +907,12 @@ hipError_t launch_reduce(hipStream_t st, const double* partials, int nblocks, do
 hipError_t launch_node_type(hipStream_t st, const LatticeDev& L, int* type, int* dirstate) {
   const int grid = (int)((L.n + 255u) / 256u);
   hipLaunchKernelGGL(k_node_type, dim3(grid), dim3(256), 0, st, L, type, dirstate);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_runs(hipStream_t st, const float* src, float* dst, size_t nblk) {
+  const size_t blocks = (nblk * 64 + 255) / 256;
+  hipLaunchKernelGGL(k_copy_runs, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, nblk);
   return hipGetLastError();
 }
 

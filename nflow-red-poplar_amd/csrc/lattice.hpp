@@ -87,6 +87,8 @@ hipError_t launch_export(int storage, hipStream_t st, const LatticeDev& L, int p
 hipError_t launch_reduce(hipStream_t st, const double* partials, int nblocks, double* out5);
 hipError_t launch_node_type(hipStream_t st, const LatticeDev& L, int* type, int* dirstate);
 hipError_t launch_copy_streams(hipStream_t st, const float* src, float* dst, size_t per_stream, int streams);
+// nblk blocks of 19 runs of 64 floats (the FP32 population layout)
+hipError_t launch_copy_runs(hipStream_t st, const float* src, float* dst, size_t nblk);
 int export_block_size();
 
 }  // namespace nflow

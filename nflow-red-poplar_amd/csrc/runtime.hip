@@ -873,15 +873,20 @@ extern "C" void nflow_batch_destroy(nflow_batch* b) { delete b; }
 
 // ---- measurement helpers -----------------------------------------------------
 extern "C" int nflow_copy_bandwidth(size_t bytes, int streams, int reps, double* gbps, double* ms_per_copy) {
-  if (streams < 1 || reps < 1 || bytes < (size_t)streams * 16) return fail(NFLOW_EINVAL, "bad copy parameters");
-  const size_t per = bytes / sizeof(float) / streams / 4 * 4;
+  // streams = 0: the FP32 population layout (blocks of 19 runs of 64 floats)
+  const bool runs = streams == 0;
+  const size_t nblk = bytes / (19 * 64 * sizeof(float));
+  if (streams < 0 || reps < 1 || (runs ? nblk == 0 : bytes < (size_t)streams * 16))
+    return fail(NFLOW_EINVAL, "bad copy parameters");
+  const int ns = runs ? 1 : streams;
+  const size_t per = runs ? nblk * 19 * 64 : bytes / sizeof(float) / streams / 4 * 4;
   float *src = nullptr, *dst = nullptr;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = 0;
   float ms = 0.0f;
-  if (hipMalloc(&src, per * streams * sizeof(float)) != hipSuccess ||
-      hipMalloc(&dst, per * streams * sizeof(float)) != hipSuccess) {
+  auto launch = [&]() { return runs ? launch_copy_runs(st, src, dst, nblk) : launch_copy_streams(st, src, dst, per, ns); };
+  if (hipMalloc(&src, per * ns * sizeof(float)) != hipSuccess || hipMalloc(&dst, per * ns * sizeof(float)) != hipSuccess) {
     rc = fail(NFLOW_ENOMEM, "copy buffers");
     goto done;
   }
@@ -890,13 +895,13 @@ extern "C" int nflow_copy_bandwidth(size_t bytes, int streams, int reps, double*
     rc = fail(NFLOW_EDEVICE, "stream/event creation");
     goto done;
   }
-  if (hipMemsetAsync(src, 0, per * streams * sizeof(float), st) != hipSuccess ||
-      launch_copy_streams(st, src, dst, per, streams) != hipSuccess || hipEventRecord(e0, st) != hipSuccess) {
+  if (hipMemsetAsync(src, 0, per * ns * sizeof(float), st) != hipSuccess || launch() != hipSuccess ||
+      hipEventRecord(e0, st) != hipSuccess) {
     rc = fail(NFLOW_EDEVICE, "copy warm-up");
     goto done;
   }
   for (int r = 0; r < reps; ++r)
-    if (launch_copy_streams(st, src, dst, per, streams) != hipSuccess) {
+    if (launch() != hipSuccess) {
       rc = fail(NFLOW_EDEVICE, "copy launch");
       goto done;
     }
@@ -906,7 +911,7 @@ extern "C" int nflow_copy_bandwidth(size_t bytes, int streams, int reps, double*
     goto done;
   }
   if (ms_per_copy) *ms_per_copy = ms / reps;
-  if (gbps) *gbps = 2.0 * (double)per * streams * sizeof(float) / ((double)ms / reps * 1e-3) / 1e9;
+  if (gbps) *gbps = 2.0 * (double)per * ns * sizeof(float) / ((double)ms / reps * 1e-3) / 1e9;
 done:
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

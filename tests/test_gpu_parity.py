@@ -151,3 +151,12 @@ def test_bad_inputs_raise():
     bad = np.array([[[0, 8], [0, 8], [0, 4]]], dtype=np.int32)  # does not cover the box
     with pytest.raises(nflow.NflowError):
         nflow.Solver(inp, bad)
+
+
+def test_copy_ceilings_run():
+    # the measurement helper in both modes: equal arrays and the FP32 population layout
+    for streams in (1, 19, 0):
+        gbs, ms = nflow.copy_bandwidth(19 * 64 * 4 * 2048, streams=streams, reps=2)
+        assert gbs > 0 and ms > 0, streams
+    with pytest.raises(nflow.NflowError):
+        nflow.copy_bandwidth(100, streams=0, reps=1)  # smaller than one 19-run block
