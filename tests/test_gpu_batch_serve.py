@@ -86,10 +86,11 @@ def test_batched_cavities_against_oracle():
         assert np.array_equal(b.read_macro(i)[1], s.read_macro()[1], equal_nan=True), i
 
 
-def test_server_results_equal_solo_results():
-    inputs = mixed_inputs(nflow.F16)
+@pytest.mark.parametrize("storage", [nflow.F16, nflow.F32, nflow.BF16])
+def test_server_results_equal_solo_results(storage):
+    inputs = mixed_inputs(storage)
     steps = [30, 5, 41, 1, 17]
-    srv = nflow.Server(budget_bytes=512 << 20, storage=nflow.F16, max_resident=3, steps_per_tick=8)
+    srv = nflow.Server(budget_bytes=512 << 20, storage=storage, max_resident=3, steps_per_tick=8)
     for i, (inp, n) in enumerate(zip(inputs, steps)):
         srv.submit(inp, n, tag=100 + i, want_fields=True)
         time.sleep(0.002 * i)
