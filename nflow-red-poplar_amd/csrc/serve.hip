@@ -110,7 +110,8 @@ class Server {
 
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<Pending> queue;
+  std::deque<Pending> queue;    // submitted, not yet seen by the worker (under mu)
+  std::deque<Pending> waiting;  // seen by the worker, not admitted (worker only)
   std::deque<nflow_result> results;
   std::map<unsigned long long, std::vector<float>> fields;
   std::vector<Resident> resident;  // parallel to batch.lats
@@ -124,33 +125,49 @@ class Server {
 
   void loop();
   void admit(double t);
+  void reject(const Pending& p, int status, double t);
   int tick();
-  void finish(size_t i, double t);
+  void finish(size_t i);
 };
 
-void Server::admit(double t) {
+void Server::reject(const Pending& p, int status, double t) {
+  nflow_result r{};
+  r.tag = p.req.tag;
+  r.status = status;
+  r.nx = p.req.input.nx;
+  r.ny = p.req.input.ny;
+  r.nz = p.req.input.nz;
+  r.submit_s = p.submit_s;
+  r.admit_s = r.done_s = t;
   std::lock_guard<std::mutex> lk(mu);
+  results.push_back(r);
+  st.rejected++;
+  cv_done.notify_all();
+}
+
+// Moves new submissions behind the ones already waiting, then admits in
+// FIFO order with backfill.  Only the lock-protected hand-over touches the
+// submitters' queue; lattice set-up (device work) runs without the lock, so
+// submit and poll never wait for the device.
+void Server::admit(double t) {
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    while (!queue.empty()) {
+      waiting.push_back(queue.front());
+      queue.pop_front();
+    }
+  }
   bool head_blocked = false;
-  for (auto it = queue.begin(); it != queue.end();) {
+  for (auto it = waiting.begin(); it != waiting.end();) {
     if ((int)resident.size() >= max_resident) break;
     const size_t bytes = lattice_bytes(it->req.input, batch.storage);
     if (bytes > arena.size()) {  // can never fit
-      nflow_result r{};
-      r.tag = it->req.tag;
-      r.status = NFLOW_EBUDGET;
-      r.nx = it->req.input.nx;
-      r.ny = it->req.input.ny;
-      r.nz = it->req.input.nz;
-      r.submit_s = it->submit_s;
-      r.admit_s = r.done_s = t;
-      results.push_back(r);
-      st.rejected++;
-      it = queue.erase(it);
-      cv_done.notify_all();
+      reject(*it, NFLOW_EBUDGET, t);
+      it = waiting.erase(it);
       continue;
     }
-    const bool is_head = it == queue.begin();
-    if (head_blocked && t - queue.front().submit_s > max_head_wait_s) break;  // no overtaking an old head
+    const bool is_head = it == waiting.begin();
+    if (head_blocked && t - waiting.front().submit_s > max_head_wait_s) break;  // no overtaking an old head
     void* mem = arena.alloc(bytes);
     if (!mem) {
       if (is_head) head_blocked = true;
@@ -161,28 +178,22 @@ void Server::admit(double t) {
     int rc = batch.add_lattice(it->req.input, {}, mem);
     if (!rc) rc = batch.init_lattice(idx, nullptr);
     if (rc) {
-      if ((int)batch.lats.size() > idx) batch.lats.pop_back();
+      if ((int)batch.lats.size() > idx) batch.remove_lattice(idx);
       arena.release(mem, bytes);
-      nflow_result r{};
-      r.tag = it->req.tag;
-      r.status = rc;
-      r.submit_s = it->submit_s;
-      r.admit_s = r.done_s = t;
-      results.push_back(r);
-      st.rejected++;
-      it = queue.erase(it);
-      cv_done.notify_all();
+      reject(*it, rc, t);
+      it = waiting.erase(it);
       continue;
     }
     resident.push_back(Resident{it->req, it->submit_s, t, (long long)it->req.nsteps});
+    it = waiting.erase(it);
+    std::lock_guard<std::mutex> lk(mu);
     st.admitted++;
     st.peak_resident = std::max<int>(st.peak_resident, (int)resident.size());
     st.peak_bytes = std::max(st.peak_bytes, arena.used());
-    it = queue.erase(it);
   }
 }
 
-void Server::finish(size_t i, double t) {
+void Server::finish(size_t i) {
   HostLattice& h = batch.lats[i];
   Resident& r = resident[i];
   nflow_result res{};
@@ -210,7 +221,6 @@ void Server::finish(size_t i, double t) {
   res.mom[2] = d[3];
   res.drho_l1 = d[4];
   res.done_s = now();
-  (void)t;
   const long long upd = (long long)h.dev.n * h.steps_done;
   const bool keep_fields = r.req.want_fields && rc == 0;
   void* mem = h.mem;
@@ -253,7 +263,7 @@ void Server::loop() {
     {
       std::unique_lock<std::mutex> lk(mu);
       if (stop) break;  // destroy abandons queued and resident work
-      if (resident.empty() && queue.empty()) {
+      if (resident.empty() && queue.empty() && waiting.empty()) {
         if (busy_from >= 0.0) {
           st.busy_s += now() - busy_from;
           busy_from = -1.0;
@@ -285,21 +295,22 @@ void Server::loop() {
         res.done_s = now();
         results.push_back(res);
       }
-      for (auto& p : queue) {
-        nflow_result res{};
-        res.tag = p.req.tag;
-        res.status = rc;
-        res.submit_s = p.submit_s;
-        results.push_back(res);
-      }
+      for (const auto* q : {&waiting, &queue})
+        for (const auto& p : *q) {
+          nflow_result res{};
+          res.tag = p.req.tag;
+          res.status = rc;
+          res.submit_s = p.submit_s;
+          results.push_back(res);
+        }
+      waiting.clear();
       queue.clear();
       resident.clear();
       cv_done.notify_all();
       break;
     }
-    const double t2 = now();
     for (size_t i = resident.size(); i-- > 0;)
-      if (batch.lats[i].steps_done >= resident[i].target) finish(i, t2);
+      if (batch.lats[i].steps_done >= resident[i].target) finish(i);
   }
   if (busy_from >= 0.0) {
     std::lock_guard<std::mutex> lk(mu);
