@@ -145,8 +145,8 @@ const char* nflow_last_error(void);
 
 /* ---- restart files (Inputval.irestart/irst, struct.h:84; kernel.cu:706) ----
  * A checkpoint is the exported arF plus the step count: loading it into a
- * solver of the same extents continues the run exactly where it was saved
- * (bitwise for FP32 storage). */
+ * solver of the same extents and case type continues the run exactly where it
+ * was saved (bitwise for FP32 storage); any other solver is refused. */
 int nflow_save_checkpoint(nflow_solver* s, const char* path);
 int nflow_load_checkpoint(nflow_solver* s, const char* path);
 int nflow_checkpoint_info(const char* path, int* nx, int* ny, int* nz, long long* steps);
@@ -237,6 +237,11 @@ void nflow_server_destroy(nflow_server* s);
  * copies in the FP32 population layout instead: blocks of 19 runs of 64
  * floats, one 4-byte element of every run per thread. */
 int nflow_copy_bandwidth(size_t bytes, int streams, int reps, double* gbps, double* ms_per_copy);
+/* VALU issue ceiling: independent FP32 FMA chains (no memory traffic) on
+ * 16 workgroups of 256 lanes per CU, timed with HIP events over `reps`
+ * launches.  *g_wave_fma_per_s = wave-level v_fma_f32 instructions per
+ * second in units of 1e9 (each one 64 lane FMAs). */
+int nflow_valu_rate(int reps, double* g_wave_fma_per_s, double* ms_per_launch);
 int nflow_device_info(char* name, int name_len, int* cu_count, size_t* hbm_bytes);
 
 #ifdef __cplusplus

@@ -606,8 +606,8 @@ extern "C" int nflow_create(const nflow_input* in, int ntile, const int slice[][
   nflow_solver* s = new nflow_solver();
   s->b.storage = in->storage;
   // one lattice: 512-node work items suit 16-bit storage (256^3 MRT-LES 2-5%
-  // ahead of 256; FP32 within 1%); batches and the server keep 256, which
-  // measured better on mixed sizes
+  // ahead of 256; FP32 within 1%); batches start at 256 and are autotuned,
+  // the server fixes its own shape
   if (in->storage != NFLOW_STORAGE_F32) s->b.tuning.block = 512;
   rc = s->b.open();
   if (!rc) rc = s->b.add_lattice(*in, tiles, nullptr);
@@ -777,6 +777,8 @@ extern "C" int nflow_load_checkpoint(nflow_solver* s, const char* path) {
   if (!rc && (h.nx != L.in.nx || h.ny != L.in.ny || h.nz != L.in.nz))
     rc = fail(NFLOW_EINVAL, "%s holds a %dx%dx%d lattice, the solver is %dx%dx%d", path, h.nx, h.ny, h.nz, L.in.nx,
               L.in.ny, L.in.nz);
+  if (!rc && h.casetype != L.in.casetype)
+    rc = fail(NFLOW_EINVAL, "%s holds casetype %d, the solver is casetype %d", path, h.casetype, L.in.casetype);
   std::vector<float> a;
   if (!rc) {
     a.resize(padded_count(L.in));
@@ -918,6 +920,51 @@ done:
   if (st) (void)hipStreamDestroy(st);
   (void)hipFree(src);
   (void)hipFree(dst);
+  return rc;
+}
+
+extern "C" int nflow_valu_rate(int reps, double* g_wave_fma_per_s, double* ms_per_launch) {
+  if (reps < 1) return fail(NFLOW_EINVAL, "reps must be >= 1");
+  int dev = 0, cus = 0;
+  NFLOW_HIP(hipGetDevice(&dev));
+  NFLOW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int grid = 16 * cus, iters = 2048;
+  float* out = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  float ms = 0.0f;
+  if (hipMalloc(&out, (size_t)grid * 256 * sizeof(float)) != hipSuccess) {
+    rc = fail(NFLOW_ENOMEM, "probe output");
+    goto done;
+  }
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(NFLOW_EDEVICE, "stream/event creation");
+    goto done;
+  }
+  if (launch_fma_chains(st, out, grid, iters) != hipSuccess || hipEventRecord(e0, st) != hipSuccess) {
+    rc = fail(NFLOW_EDEVICE, "probe warm-up");
+    goto done;
+  }
+  for (int r = 0; r < reps; ++r)
+    if (launch_fma_chains(st, out, grid, iters) != hipSuccess) {
+      rc = fail(NFLOW_EDEVICE, "probe launch");
+      goto done;
+    }
+  if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+      hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+    rc = fail(NFLOW_EDEVICE, "probe timing");
+    goto done;
+  }
+  if (ms_per_launch) *ms_per_launch = ms / reps;
+  if (g_wave_fma_per_s)  // grid * 4 waves, 64 FMAs per wave per iteration
+    *g_wave_fma_per_s = (double)grid * 4 * 64 * iters / ((double)ms / reps * 1e-3) / 1e9;
+done:
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(out);
   return rc;
 }
 
