@@ -37,6 +37,12 @@ WORKLOADS = {
     "cavity256_bf16": dict(nx=256, ny=256, nz=256, casetype=1, storage=2,
                            desc="3D lid-driven cavity 256^3, D3Q19 MRT + LES (Cs 0.17), tau 0.6, BF16 storage "
                                 "of f - w_q, FP32 arithmetic"),
+    "slab256x32": dict(nx=256, ny=32, nz=256, casetype=1, storage=0,
+                       desc="one GPU's share of a 256^3 cavity cut into 8 y-slabs: 256x32x256, D3Q19 MRT + LES "
+                            "(Cs 0.17), tau 0.6, FP32 populations (slab faces run the wall rules)"),
+    "slab256x32_f16": dict(nx=256, ny=32, nz=256, casetype=1, storage=1,
+                           desc="one GPU's share of a 256^3 cavity cut into 8 y-slabs: 256x32x256, D3Q19 MRT + "
+                                "LES (Cs 0.17), tau 0.6, FP16 storage of f - w_q (slab faces run the wall rules)"),
     "channel": dict(nx=512, ny=128, nz=128, casetype=0, storage=0,
                     desc="square-duct channel 512x128x128, D3Q19 MRT + LES (Cs 0.17), tau 0.56, parabolic "
                          "inlet peak 0.05 with seeded 1% perturbation, pressure outlet, FP32 populations"),
@@ -60,6 +66,17 @@ def traffic_from_profiles(name):
     with open(path) as f:
         d = json.load(f)
     return d.get("bytes_per_launch")
+
+
+def valu_per_node_from_profiles(name):
+    """k_step VALU instructions per node (SQ_INSTS_VALU / SQ_WAVES / 64 lanes,
+    one node per lane) measured by rocprofv3 --pmc, if recorded."""
+    path = os.path.join(ROOT, "profiles", f"instructions_{name}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("k_step", {}).get("valu_per_node")
 
 
 def cpu_model():
@@ -266,22 +283,45 @@ def bench_precision(args):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    cases = [
-        ("channel 512x128x128 MRT-LES Cs 0.17 tau 0.56, inlet peak 0.05 + 1% seeded", args.channel_steps,
-         dict(nx=512, ny=128, nz=128, casetype=0, enable_mrt=1, enable_les=1, smagorinsky=0.17, tau=0.56,
-              inlet_peak=0.05, perturb=0.01, seed=2024, ref_vel=(0.05, 0.0, 0.0), init_lid_velocity=0)),
-        ("cavity 128^3 MRT-LES Cs 0.17 tau 0.6, lid 0.05", args.cavity_steps,
-         dict(nx=128, ny=128, nz=128, casetype=1, enable_mrt=1, enable_les=1, smagorinsky=0.17, tau=0.6,
-              ref_vel=(0.05, 0.0, 0.0), init_lid_velocity=0)),
-    ]
+    cases = []
+    want = args.cases.split(",")
+    if "channel" in want:
+        cases.append(("channel 512x128x128 MRT-LES Cs 0.17 tau 0.56, inlet peak 0.05 + 1% seeded", args.channel_steps,
+                      dict(nx=512, ny=128, nz=128, casetype=0, enable_mrt=1, enable_les=1, smagorinsky=0.17,
+                           tau=0.56, inlet_peak=0.05, perturb=0.01, seed=2024, ref_vel=(0.05, 0.0, 0.0),
+                           init_lid_velocity=0)))
+    if "cavity" in want:
+        cases.append((f"cavity 128^3 MRT-LES Cs 0.17 tau {args.cavity_tau:g}, lid 0.05", args.cavity_steps,
+                      dict(nx=128, ny=128, nz=128, casetype=1, enable_mrt=1, enable_les=1, smagorinsky=0.17,
+                           tau=args.cavity_tau, ref_vel=(0.05, 0.0, 0.0), init_lid_velocity=0)))
+
+    def oracle_macro(c, steps, fp32):
+        L = oracle.Lattice(oracle.default_params(**c), fp32=fp32)
+        L.initialize()
+        done = 0
+        while done < steps:  # in chunks, with a progress line each
+            k = min(250, steps - done)
+            L.run(k)
+            done += k
+            print(f"oracle{'-f32' if fp32 else ''} {c['nx']}x{c['ny']}x{c['nz']}: {done}/{steps} steps",
+                  file=sys.stderr, flush=True)
+        m = [np.asarray(a, dtype=np.float64) for a in L.read_macro()]
+        del L
+        return m
+
+    def errors(r, u, ro, uo):
+        return {"rel_l2_u": float(np.linalg.norm(u - uo) / np.linalg.norm(uo)),
+                "rel_l2_rho": float(np.linalg.norm(r - ro) / np.linalg.norm(ro)),
+                "rel_l2_rho_minus_1": float(np.linalg.norm(r - ro) / np.linalg.norm(ro - 1.0))}
+
     table = []
     for name, steps, c in cases:
-        L = oracle.Lattice(oracle.default_params(**c))
-        L.initialize()
-        L.run(steps)
-        ro, xo, yo, zo = L.read_macro()
-        del L
+        ro, xo, yo, zo = oracle_macro(c, steps, False)
         uo = np.stack([xo, yo, zo])
+        if args.f32_oracle:  # the FP32 round-off floor: the oracle's own algorithm in FP32
+            r3, x3, y3, z3 = oracle_macro(c, steps, True)
+            table.append({"case": name, "steps": steps, "storage": "oracle-f32",
+                          **errors(r3, np.stack([x3, y3, z3]), ro, uo)})
         for storage, sname in ((nflow.F32, "f32"), (nflow.F16, "f16-shifted"), (nflow.BF16, "bf16-shifted")):
             b = nflow.Batch([nflow.defaults(storage=storage, **c)])
             b.set_tuning(256 if storage else 512, 0, 0, 10)
@@ -292,10 +332,7 @@ def bench_precision(args):
             b.close()
             u = np.stack([ux, uy, uz]).astype(np.float64)
             r = r.astype(np.float64)
-            table.append({"case": name, "steps": steps, "storage": sname,
-                          "rel_l2_u": float(np.linalg.norm(u - uo) / np.linalg.norm(uo)),
-                          "rel_l2_rho": float(np.linalg.norm(r - ro) / np.linalg.norm(ro)),
-                          "rel_l2_rho_minus_1": float(np.linalg.norm(r - ro) / np.linalg.norm(ro - 1.0)),
+            table.append({"case": name, "steps": steps, "storage": sname, **errors(r, u, ro, uo),
                           "mlups": round(c["nx"] * c["ny"] * c["nz"] / (ms * 1e-3) / 1e6, 1)})
     print(json.dumps({"metric": "rel-L2 error of rho and u against the FP64 oracle per storage format",
                       "unit": "relative L2", "n_gpus": 1, "data": "synthetic", "results": table}))
@@ -310,6 +347,9 @@ def main():
     ap.add_argument("--mode", default="single", choices=("single", "batch", "serve", "precision"))
     ap.add_argument("--channel-steps", type=int, default=200, help="precision mode: channel steps")
     ap.add_argument("--cavity-steps", type=int, default=1000, help="precision mode: cavity steps")
+    ap.add_argument("--cavity-tau", type=float, default=0.6, help="precision mode: cavity relaxation time")
+    ap.add_argument("--cases", default="channel,cavity", help="precision mode: comma-separated channel,cavity")
+    ap.add_argument("--f32-oracle", action="store_true", help="precision mode: also the FP32 build of the oracle")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true", help="fixed launch shape instead of the autotuner's")
     ap.add_argument("--graph-steps", type=int, default=10)
@@ -369,6 +409,12 @@ def main():
     copy1_gbs, _ = nflow.copy_bandwidth(2 * 19 * 4 * 4_000_000, streams=1, reps=20)
     copy_gbs, _ = nflow.copy_bandwidth(2 * 19 * 4 * 4_000_000, streams=19, reps=20)
     copy_runs_gbs, _ = nflow.copy_bandwidth(2 * 19 * 4 * 4_000_000, streams=0, reps=20)
+    fma_g, _ = nflow.valu_rate(reps=5)
+    vpn = valu_per_node_from_profiles(args.workload)
+    valu = {"fma_chain_g_wave_per_s": round(fma_g, 1), "k_step_valu_per_node": vpn,
+            "k_step_g_wave_per_s": None if vpn is None else round(vpn * n / 64 / (dev_ms * 1e-3) / 1e9, 1)}
+    if vpn is not None:
+        valu["k_step_over_fma_chain"] = round(valu["k_step_g_wave_per_s"] / fma_g, 3)
     out = {
         "metric": METRIC,
         "value": round(mlups, 1),
@@ -391,6 +437,7 @@ def main():
         "hbm_frac_of_peak": round(achieved / HBM_PEAK_GBS, 4),
         "copy_ceiling_gbs": {"1_stream": round(copy1_gbs, 1), "19_streams": round(copy_gbs, 1),
                              "population_runs": round(copy_runs_gbs, 1)},
+        "valu_issue": valu,
         "mass_drift": abs(m - n) / n,
     }
     b.close()

@@ -822,6 +822,25 @@ __global__ void __launch_bounds__(256) k_copy_runs(const float* __restrict__ src
   for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(v[q], (gptr<float>)dst + (blk * Q + q) * 64 + lane);
 }
 
+// VALU issue ceiling: eight independent v_fma_f32 chains per lane, 64 FMAs
+// per loop iteration, no memory traffic until the single store at the end.
+__global__ void __launch_bounds__(256) k_fma_chains(float* __restrict__ out, float a, float b, int iters) {
+  float x[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) x[c] = (float)(threadIdx.x + c);
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) x[c] = __builtin_fmaf(x[c], a, b);
+    }
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) s += x[c];
+  out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -913,6 +932,11 @@ hipError_t launch_node_type(hipStream_t st, const LatticeDev& L, int* type, int*
 hipError_t launch_copy_runs(hipStream_t st, const float* src, float* dst, size_t nblk) {
   const size_t blocks = (nblk * 64 + 255) / 256;
   hipLaunchKernelGGL(k_copy_runs, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, nblk);
+  return hipGetLastError();
+}
+
+hipError_t launch_fma_chains(hipStream_t st, float* out, int grid, int iters) {
+  hipLaunchKernelGGL(k_fma_chains, dim3((unsigned)grid), dim3(256), 0, st, out, 0.999f, 0.001f, iters);
   return hipGetLastError();
 }
 

@@ -89,6 +89,9 @@ hipError_t launch_node_type(hipStream_t st, const LatticeDev& L, int* type, int*
 hipError_t launch_copy_streams(hipStream_t st, const float* src, float* dst, size_t per_stream, int streams);
 // nblk blocks of 19 runs of 64 floats (the FP32 population layout)
 hipError_t launch_copy_runs(hipStream_t st, const float* src, float* dst, size_t nblk);
+// grid workgroups of 256 lanes, each lane 64 * iters independent FMAs (out:
+// grid * 256 floats)
+hipError_t launch_fma_chains(hipStream_t st, float* out, int grid, int iters);
 int export_block_size();
 
 }  // namespace nflow

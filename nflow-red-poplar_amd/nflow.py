@@ -119,6 +119,7 @@ SIGNATURES = {
     "nflow_server_stats_get": (_I, [_VP, ctypes.POINTER(ServerStats)]),
     "nflow_server_destroy": (None, [_VP]),
     "nflow_copy_bandwidth": (_I, [_SZ, _I, _I, _DP, _DP]),
+    "nflow_valu_rate": (_I, [_I, _DP, _DP]),
     "nflow_device_info": (_I, [ctypes.c_char_p, _I, _IP, ctypes.POINTER(ctypes.c_size_t)]),
 }
 
@@ -432,6 +433,14 @@ class Server:
 def copy_bandwidth(nbytes, streams=19, reps=10):
     g, ms = ctypes.c_double(), ctypes.c_double()
     _check(lib().nflow_copy_bandwidth(int(nbytes), streams, reps, ctypes.byref(g), ctypes.byref(ms)))
+    return g.value, ms.value
+
+
+def valu_rate(reps=5):
+    """Wave-level FP32 FMA instructions per second (1e9) of independent FMA
+    chains, and the milliseconds per probe launch."""
+    g, ms = ctypes.c_double(), ctypes.c_double()
+    _check(lib().nflow_valu_rate(reps, ctypes.byref(g), ctypes.byref(ms)))
     return g.value, ms.value
 
 

@@ -160,3 +160,10 @@ def test_copy_ceilings_run():
         assert gbs > 0 and ms > 0, streams
     with pytest.raises(nflow.NflowError):
         nflow.copy_bandwidth(100, streams=0, reps=1)  # smaller than one 19-run block
+
+
+def test_valu_probe_runs():
+    g, ms = nflow.valu_rate(reps=2)
+    assert g > 0 and ms > 0
+    with pytest.raises(nflow.NflowError):
+        nflow.valu_rate(reps=0)
