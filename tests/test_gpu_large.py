@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import nflow
-from helpers import cfg, fp32_parity, run_gpu
+from helpers import cfg, fp32_parity, gpu_input, run_gpu
 
 pytestmark = pytest.mark.gpu
 
@@ -48,3 +48,32 @@ def test_cavity_256_is_deterministic(storage):
     b = run_gpu(c, 120, storage=storage).read_macro()
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("storage", [nflow.F32, nflow.F16])
+def test_largest_accepted_lattice_matches_a_short_box(storage):
+    """512 x 862 x 512 (226 M nodes) is the largest box of that x-z extent the
+    32-bit element indexing accepts (19 round_up(n, 128) < 2^32; one more
+    plane is refused).  From rest, two steps disturb only the three planes
+    under the lid (the state moves one plane per step), so those must equal,
+    bitwise, the top planes of an 8-plane box of the same x-z extent, and
+    every other plane must equal that box's untouched bottom plane; the FP64
+    mass sums must differ by exactly the added rest planes."""
+    nx, ny, nz = 512, 862, 512
+    assert 19 * -(-nx * ny * nz // 128) * 128 < 2 ** 32 <= 19 * -(-nx * (ny + 1) * nz // 128) * 128
+    c = dict(nx=nx, nz=nz, enable_les=1, smagorinsky=0.17, tau=0.6)
+    small = run_gpu(cfg(ny=8, **c), 2, storage=storage)
+    ms = small.diagnostics()[0]
+    sm = small.read_macro()
+    node_mass = float(np.sum(small.read_arF()[1, 1, 1].astype(np.float64)))  # a rest node, as the FP64 sum sees it
+    small.close()
+    with nflow.Solver(gpu_input(cfg(ny=ny, **c), storage)) as big:
+        big.initialize()
+        big.run(2)
+        mb = big.diagnostics()[0]
+        bm = big.read_macro()
+    for a, b in zip(bm, sm):  # [ny][nz][nx]
+        assert np.array_equal(a[-3:], b[-3:])
+        assert (a[:-3] == b[0]).all()
+    assert abs(mb - (ms + (ny - 8) * nx * nz * node_mass)) < 1e-9 * mb
+
