@@ -200,7 +200,8 @@ def bench_serve(args):
     import workloads
 
     reqs = workloads.request_stream(seed=args.seed, count=args.requests, mean_gap_s=0.05,
-                                    step_scale=args.step_scale, storage=nflow.F16)
+                                    channel_fraction=args.channel_fraction, step_scale=args.step_scale,
+                                    storage=nflow.F16)
     _, _, hbm = nflow.device_info()
     budget = min(200 << 30, int(hbm * 0.75))
     srv = nflow.Server(budget, storage=nflow.F16, max_resident=64, steps_per_tick=args.tick)
@@ -256,9 +257,10 @@ def bench_serve(args):
         "metric": "served-batch aggregate MLUPS and p50/p99 latency", "value": round(mlups, 1), "unit": "MLUPS",
         "n_gpus": 1, "higher_is_better": True, "dtype": "f16-storage/f32-compute",
         "data": f"synthetic Poisson stream, seed {args.seed}",
-        "config": {"workload": "Poisson arrivals mean 50 ms; 1/2 cavity edge U[48,256] Re logU[100,5000] MRT-LES, "
-                               "1/2 channel length U[256,1024] x 128 x 128 MRT-LES; steps U[500,20000] x "
-                               f"{args.step_scale}", "requests": len(reqs), "budget_bytes": budget,
+        "config": {"workload": f"Poisson arrivals mean 50 ms; {1 - args.channel_fraction:g} cavity edge U[48,256] "
+                               "Re logU[100,5000] MRT-LES, "
+                               f"{args.channel_fraction:g} channel length U[256,1024] x 128 x 128 MRT-LES; "
+                               f"steps U[500,20000] x {args.step_scale}", "requests": len(reqs), "budget_bytes": budget,
                    "steps_per_tick": args.tick},
         "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
         "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
@@ -358,6 +360,7 @@ def main():
     ap.add_argument("--requests", type=int, default=48, help="serve mode: length of the request stream")
     ap.add_argument("--step-scale", type=float, default=0.1, help="serve mode: scale of the per-request steps")
     ap.add_argument("--tick", type=int, default=100, help="serve mode: steps per tick")
+    ap.add_argument("--channel-fraction", type=float, default=0.5, help="serve mode: share of channel requests")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab decomposition is "
