@@ -145,3 +145,61 @@ def test_server_backfills_behind_a_blocked_request():
     assert res[3].admit_s < res[1].done_s  # backfilled while 1 was resident
     assert res[2].admit_s >= res[1].done_s - 1e-3
     srv.close()
+
+
+def test_server_concurrent_submitters():
+    """Four host threads submit while the worker runs and the main thread
+    polls; every request completes with the mass its solo run gives (FP32,
+    bitwise), and the counters add up."""
+    import threading
+
+    inputs = mixed_inputs(nflow.F32)
+    plan = [(t, k, (t * 6 + k) % len(inputs), 3 + (t + 2 * k) % 9) for t in range(4) for k in range(6)]
+    srv = nflow.Server(budget_bytes=256 << 20, storage=nflow.F32, max_resident=4, steps_per_tick=5)
+    errors = []
+
+    def submit(t):
+        try:
+            for tt, k, i, n in plan:
+                if tt == t:
+                    srv.submit(inputs[i], n, tag=1000 * (t + 1) + k)
+                    time.sleep(0.001)
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=submit, args=(t,)) for t in range(4)]
+    for th in threads:
+        th.start()
+    res = {}
+    deadline = time.time() + 60
+    while len(res) < len(plan) and time.time() < deadline:
+        for r in srv.poll(64, wait_ms=100):
+            res[r.tag] = r
+    for th in threads:
+        th.join()
+    assert not errors and len(res) == len(plan)
+    solo = {}
+    for t, k, i, n in plan:
+        r = res[1000 * (t + 1) + k]
+        assert r.status == 0 and r.steps == n
+        if (i, n) not in solo:
+            s = nflow.Solver(inputs[i])
+            s.initialize()
+            s.run(n)
+            solo[(i, n)] = s.diagnostics()[0]
+            s.close()
+        assert r.mass == solo[(i, n)], (t, k)
+    st = srv.stats()
+    assert st["submitted"] == st["completed"] == len(plan) and st["peak_resident"] <= 4
+    srv.close()
+
+
+def test_server_close_abandons_queued_work_promptly():
+    big = nflow.defaults(**cfg(nx=64, ny=64, nz=64))
+    srv = nflow.Server(budget_bytes=64 << 20, storage=nflow.F16, max_resident=2, steps_per_tick=4)
+    for i in range(6):
+        srv.submit(big, 1_000_000, tag=i)
+    time.sleep(0.2)
+    t0 = time.time()
+    srv.close()  # the worker finishes its current tick and exits
+    assert time.time() - t0 < 10
