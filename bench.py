@@ -222,6 +222,8 @@ def bench_serve(args):
     while len(results) < len(reqs):
         for r in srv.poll(64, wait_ms=200):
             results[r.tag] = dict(status=r.status, submit=r.submit_s, admit=r.admit_s, done=r.done_s)
+            if len(results) % 8 == 0:
+                print(f"served {len(results)}/{len(reqs)}", file=sys.stderr, flush=True)
     th.join()
     wall = max(v["done"] for v in results.values()) - min(v["submit"] for v in results.values())
     st = srv.stats()
@@ -234,7 +236,9 @@ def bench_serve(args):
     # the same requests, one at a time, plain FP32 solver
     serial = 0.0
     errs = []
-    for r in reqs:
+    for k, r in enumerate(reqs):
+        if k % 8 == 0:
+            print(f"serial {k}/{len(reqs)}", file=sys.stderr, flush=True)
         inp = r["input"]
         plain = nflow.defaults(**{k: getattr(inp, k) for k, _ in inp._fields_ if k != "ref_vel"},
                                ref_vel=tuple(inp.ref_vel))
