@@ -205,7 +205,12 @@ def bench_serve(args):
     _, _, hbm = nflow.device_info()
     budget = min(200 << 30, int(hbm * 0.75))
     srv = nflow.Server(budget, storage=nflow.F16, max_resident=64, steps_per_tick=args.tick)
+    srv.set_tau_floor(args.tau_floor)
+    # accuracy checks against FP32 solo runs: the four smallest requests and the
+    # four lowest-tau ones that stay in 16-bit storage
     check = sorted(reqs, key=lambda r: r["nodes"])[:4]
+    check += [r for r in sorted(reqs, key=lambda r: r["input"].tau)
+              if r["input"].tau >= args.tau_floor and r not in check][:4]
     check_tags = {r["tag"] for r in check}
     t_start = time.perf_counter()
 
@@ -254,7 +259,9 @@ def bench_serve(args):
             g = fields[r["tag"]]
             u32 = np.stack([ux, uy, uz]).astype(np.float64)
             u16 = np.stack(g[1:]).astype(np.float64)
-            errs.append(float(np.linalg.norm(u16 - u32) / np.linalg.norm(u32)))
+            errs.append({"request": r["kind"], "tau": round(inp.tau, 5), "steps": r["steps"],
+                         "fp32_storage": inp.tau < args.tau_floor,
+                         "rel_l2_u": float(np.linalg.norm(u16 - u32) / np.linalg.norm(u32))})
     mlups = updates / wall / 1e6
     serial_mlups = updates / serial / 1e6
     print(json.dumps({
@@ -273,7 +280,10 @@ def bench_serve(args):
         "peak_resident": st["peak_resident"], "peak_bytes": st["peak_bytes"],
         "server_busy_s": round(st["busy_s"], 3), "server_step_s": round(st["step_s"], 3), "ticks": st["ticks"],
         "step_mlups": round(st["node_updates"] / max(st["step_s"], 1e-9) / 1e6, 1),
-        "rel_l2_u_vs_fp32_solo": [round(e, 6) for e in errs],
+        "tau_floor": args.tau_floor,
+        "fp32_requests": sum(1 for r in reqs if r["input"].tau < args.tau_floor),
+        "rel_l2_u_vs_fp32_solo": errs,
+        "max_rel_l2_u_checked": max(e["rel_l2_u"] for e in errs),
         "latency_includes": "queueing: offered load exceeds capacity",
     }))
 
@@ -365,6 +375,8 @@ def main():
     ap.add_argument("--step-scale", type=float, default=0.1, help="serve mode: scale of the per-request steps")
     ap.add_argument("--tick", type=int, default=100, help="serve mode: steps per tick")
     ap.add_argument("--channel-fraction", type=float, default=0.5, help="serve mode: share of channel requests")
+    ap.add_argument("--tau-floor", type=float, default=0.0,
+                    help="serve mode: requests with tau below this are kept in FP32 storage")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab decomposition is "

@@ -218,12 +218,17 @@ typedef struct nflow_server_stats {
   double busy_s;            /* wall time with at least one resident lattice */
   size_t peak_bytes;        /* peak device bytes held by resident lattices */
   int peak_resident;
-  long long ticks;          /* step launches issued as one batch between host visits */
-  double step_s;            /* device time of those launches (HIP events around each tick) */
+  long long ticks;          /* rounds of step launches between host visits */
+  double step_s;            /* host time from the first launch of a tick to its completion, summed */
 } nflow_server_stats;
 
 int nflow_server_create(size_t hbm_budget_bytes, int storage, int max_resident, int steps_per_tick,
                         nflow_server** out);
+/* Requests submitted afterwards whose relaxation time tau lies below tau_min
+ * are kept in FP32 storage (a second batch, its own launch per step) on a
+ * server whose storage is FP16/BF16: near tau = 0.5 the 16-bit rounding of
+ * f - w_q is weakly damped and accumulates.  0 (the default) turns it off. */
+int nflow_server_set_tau_floor(nflow_server* s, double tau_min);
 int nflow_server_submit(nflow_server* s, const nflow_request* r);
 int nflow_server_poll(nflow_server* s, nflow_result* out, int max_results, int wait_ms);
 int nflow_server_fetch_macro(nflow_server* s, unsigned long long tag, float* rho, float* ux, float* uy, float* uz);

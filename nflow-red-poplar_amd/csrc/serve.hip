@@ -11,7 +11,9 @@
 // step count; finished lattices are exported (FP64 diagnostics, optionally
 // rho/u) and their memory returns to the arena before the next admission.
 // A lattice's arithmetic never depends on what else is resident, so a
-// request gives the same result alone and in any mix.
+// request gives the same result alone and in any mix.  With a tau floor set,
+// requests whose relaxation time lies below it are kept in FP32 storage: they
+// form a second batch, stepped in its own launch on its own stream each tick.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -90,6 +92,7 @@ class Arena {
 struct Pending {
   nflow_request req;
   double submit_s;
+  int which;  // batch: 0 the server's storage, 1 FP32
 };
 
 struct Resident {
@@ -100,8 +103,9 @@ struct Resident {
 
 class Server {
  public:
-  Batch batch;
+  Batch batches[2];  // [0] the server's storage, [1] FP32 (below the tau floor)
   Arena arena;
+  double tau_floor = 0.0;
   int device = 0;
   int max_resident = 64;
   int steps_per_tick = 100;
@@ -114,7 +118,7 @@ class Server {
   std::deque<Pending> waiting;  // seen by the worker, not admitted (worker only)
   std::deque<nflow_result> results;
   std::map<unsigned long long, std::vector<float>> fields;
-  std::vector<Resident> resident;  // parallel to batch.lats
+  std::vector<Resident> resident[2];  // parallel to batches[w].lats
   nflow_server_stats st{};
   bool stop = false;
   int fatal = 0;
@@ -127,7 +131,8 @@ class Server {
   void admit(double t);
   void reject(const Pending& p, int status, double t);
   int tick();
-  void finish(size_t i);
+  void finish(int w, size_t i);
+  size_t resident_count() const { return resident[0].size() + resident[1].size(); }
 };
 
 void Server::reject(const Pending& p, int status, double t) {
@@ -159,7 +164,9 @@ void Server::admit(double t) {
   }
   bool head_blocked = false;
   for (auto it = waiting.begin(); it != waiting.end();) {
-    if ((int)resident.size() >= max_resident) break;
+    if ((int)resident_count() >= max_resident) break;
+    const int w = it->which;
+    Batch& batch = batches[w];
     const size_t bytes = lattice_bytes(it->req.input, batch.storage);
     if (bytes > arena.size()) {  // can never fit
       reject(*it, NFLOW_EBUDGET, t);
@@ -184,18 +191,19 @@ void Server::admit(double t) {
       it = waiting.erase(it);
       continue;
     }
-    resident.push_back(Resident{it->req, it->submit_s, t, (long long)it->req.nsteps});
+    resident[w].push_back(Resident{it->req, it->submit_s, t, (long long)it->req.nsteps});
     it = waiting.erase(it);
     std::lock_guard<std::mutex> lk(mu);
     st.admitted++;
-    st.peak_resident = std::max<int>(st.peak_resident, (int)resident.size());
+    st.peak_resident = std::max<int>(st.peak_resident, (int)resident_count());
     st.peak_bytes = std::max(st.peak_bytes, arena.used());
   }
 }
 
-void Server::finish(size_t i) {
+void Server::finish(int w, size_t i) {
+  Batch& batch = batches[w];
   HostLattice& h = batch.lats[i];
-  Resident& r = resident[i];
+  Resident& r = resident[w][i];
   nflow_result res{};
   res.tag = r.req.tag;
   res.nx = h.in.nx;
@@ -227,7 +235,7 @@ void Server::finish(size_t i) {
   const size_t bytes = h.bytes;
   batch.remove_lattice((int)i);
   arena.release(mem, bytes);
-  resident.erase(resident.begin() + i);  // r and h are invalid from here on
+  resident[w].erase(resident[w].begin() + i);  // r and h are invalid from here on
   std::lock_guard<std::mutex> lk(mu);
   if (keep_fields) fields[res.tag] = std::move(f);
   results.push_back(res);
@@ -236,21 +244,28 @@ void Server::finish(size_t i) {
   cv_done.notify_all();
 }
 
+// Advances every resident lattice by k steps: one launch per step for each
+// non-empty batch, the two batches on their own streams; step_s accumulates
+// the host time from the first launch to the end of the last.
 int Server::tick() {
   long long k = steps_per_tick;
-  for (size_t i = 0; i < resident.size(); ++i)
-    k = std::min(k, resident[i].target - batch.lats[i].steps_done);
+  for (int w = 0; w < 2; ++w)
+    for (size_t i = 0; i < resident[w].size(); ++i)
+      k = std::min(k, resident[w][i].target - batches[w].lats[i].steps_done);
+  const double t0 = now();
+  if (k > 0)
+    for (int w = 0; w < 2; ++w)
+      if (!resident[w].empty()) {
+        int rc = batches[w].run((int)k);
+        if (rc) return rc;
+      }
+  for (int w = 0; w < 2; ++w)
+    if (batches[w].stream) NFLOW_HIP(hipStreamSynchronize(batches[w].stream));
   if (k > 0) {
-    int rc = batch.run((int)k);  // launch_steps between the batch's timing events
-    if (rc) return rc;
-  }
-  NFLOW_HIP(hipStreamSynchronize(batch.stream));
-  if (k > 0) {
-    double ms = 0.0;
-    const bool timed = batch.timing(&ms, nullptr) == 0;
+    const double dt = now() - t0;
     std::lock_guard<std::mutex> lk(mu);  // stats are read by nflow_server_stats_get
     st.launches += k;
-    if (timed) st.step_s += ms * (double)k * 1e-3;
+    st.step_s += dt;
     st.ticks++;
   }
   return 0;
@@ -263,7 +278,7 @@ void Server::loop() {
     {
       std::unique_lock<std::mutex> lk(mu);
       if (stop) break;  // destroy abandons queued and resident work
-      if (resident.empty() && queue.empty() && waiting.empty()) {
+      if (resident_count() == 0 && queue.empty() && waiting.empty()) {
         if (busy_from >= 0.0) {
           st.busy_s += now() - busy_from;
           busy_from = -1.0;
@@ -274,7 +289,7 @@ void Server::loop() {
     }
     const double t = now();
     admit(t);
-    if (resident.empty()) {
+    if (resident_count() == 0) {
       std::unique_lock<std::mutex> lk(mu);
       cv_work.wait_for(lk, std::chrono::milliseconds(5));
       continue;
@@ -286,15 +301,16 @@ void Server::loop() {
       fatal = rc;
       fatal_msg = last_error_slot();
       // fail everything still held
-      for (auto& r : resident) {
-        nflow_result res{};
-        res.tag = r.req.tag;
-        res.status = rc;
-        res.submit_s = r.submit_s;
-        res.admit_s = r.admit_s;
-        res.done_s = now();
-        results.push_back(res);
-      }
+      for (int w = 0; w < 2; ++w)
+        for (auto& r : resident[w]) {
+          nflow_result res{};
+          res.tag = r.req.tag;
+          res.status = rc;
+          res.submit_s = r.submit_s;
+          res.admit_s = r.admit_s;
+          res.done_s = now();
+          results.push_back(res);
+        }
       for (const auto* q : {&waiting, &queue})
         for (const auto& p : *q) {
           nflow_result res{};
@@ -305,12 +321,14 @@ void Server::loop() {
         }
       waiting.clear();
       queue.clear();
-      resident.clear();
+      resident[0].clear();
+      resident[1].clear();
       cv_done.notify_all();
       break;
     }
-    for (size_t i = resident.size(); i-- > 0;)
-      if (batch.lats[i].steps_done >= resident[i].target) finish(i);
+    for (int w = 0; w < 2; ++w)
+      for (size_t i = resident[w].size(); i-- > 0;)
+        if (batches[w].lats[i].steps_done >= resident[w][i].target) finish(w, i);
   }
   if (busy_from >= 0.0) {
     std::lock_guard<std::mutex> lk(mu);
@@ -333,19 +351,22 @@ extern "C" int nflow_server_create(size_t hbm_budget_bytes, int storage, int max
   *out = nullptr;
   nflow_server* sv = new nflow_server();
   Server& s = sv->s;
-  s.batch.storage = storage;
+  s.batches[0].storage = storage;
+  s.batches[1].storage = NFLOW_STORAGE_F32;
   s.max_resident = max_resident;
   s.steps_per_tick = steps_per_tick;
-  nflow_tuning t = s.batch.tuning;
+  nflow_tuning t = s.batches[0].tuning;
   t.graph_steps = 0;  // the resident set changes between ticks
   // measured on a 48-request FP16 stream (cavities 48-256, channels
   // 256-1024 x 128 x 128): 512-node items with XCD-ordered dealing 67.8-68.4
   // GLUPS, 512 alone 67.0, 256 with XCD order 66.6, 256 alone 63.8-64.3
   t.block = 512;
   t.xcd_order = 1;
-  s.batch.tuning = t;
+  s.batches[0].tuning = t;
+  s.batches[1].tuning = t;
   int rc = hipGetDevice(&s.device) == hipSuccess ? 0 : fail(NFLOW_EDEVICE, "hipGetDevice");
-  if (!rc) rc = s.batch.open();
+  if (!rc) rc = s.batches[0].open();
+  if (!rc && storage != NFLOW_STORAGE_F32) rc = s.batches[1].open();
   if (!rc) rc = s.arena.open(hbm_budget_bytes);
   if (rc) {
     delete sv;
@@ -357,17 +378,25 @@ extern "C" int nflow_server_create(size_t hbm_budget_bytes, int storage, int max
   return 0;
 }
 
+extern "C" int nflow_server_set_tau_floor(nflow_server* sv, double tau_min) {
+  if (!sv || !(tau_min >= 0.0)) return fail(NFLOW_EINVAL, "nflow_server_set_tau_floor: bad argument");
+  std::lock_guard<std::mutex> lk(sv->s.mu);
+  sv->s.tau_floor = tau_min;
+  return 0;
+}
+
 extern "C" int nflow_server_submit(nflow_server* sv, const nflow_request* r) {
   if (!sv || !r) return fail(NFLOW_EINVAL, "nflow_server_submit: null argument");
   if (r->nsteps < 0) return fail(NFLOW_EINVAL, "nsteps must be >= 0");
   Server& s = sv->s;
-  int rc = validate_input(r->input, s.batch.storage);
+  int rc = validate_input(r->input, s.batches[0].storage);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(s.mu);
   if (s.fatal) return fail(s.fatal, "server stopped: %s", s.fatal_msg.c_str());
   if (s.stop) return fail(NFLOW_ESTATE, "server is shutting down");
-  Pending p{*r, s.now()};
-  p.req.input.storage = s.batch.storage;
+  const int w = (s.batches[0].storage != NFLOW_STORAGE_F32 && r->input.tau < s.tau_floor) ? 1 : 0;
+  Pending p{*r, s.now(), w};
+  p.req.input.storage = s.batches[w].storage;
   s.queue.push_back(p);
   s.st.submitted++;
   s.cv_work.notify_all();

@@ -112,6 +112,7 @@ SIGNATURES = {
     "nflow_batch_device_bytes": (_SZ, [_VP]),
     "nflow_batch_destroy": (None, [_VP]),
     "nflow_server_create": (_I, [_SZ, _I, _I, _I, ctypes.POINTER(_VP)]),
+    "nflow_server_set_tau_floor": (_I, [_VP, ctypes.c_double]),
     "nflow_server_submit": (_I, [_VP, ctypes.POINTER(Request)]),
     "nflow_server_poll": (_I, [_VP, ctypes.POINTER(Result), _I, _I]),
     "nflow_server_fetch_macro": (_I, [_VP, ctypes.c_ulonglong, _FP, _FP, _FP, _FP]),
@@ -406,6 +407,10 @@ class Server:
             self.h = None
 
     __del__ = close
+
+    def set_tau_floor(self, tau_min):
+        """Requests with tau below tau_min are kept in FP32 storage (0: off)."""
+        _check(lib().nflow_server_set_tau_floor(self.h, float(tau_min)))
 
     def submit(self, inp, nsteps, tag, want_fields=False):
         r = Request(inp, nsteps, 1 if want_fields else 0, tag)

@@ -203,3 +203,36 @@ def test_server_close_abandons_queued_work_promptly():
     t0 = time.time()
     srv.close()  # the worker finishes its current tick and exits
     assert time.time() - t0 < 10
+
+
+@pytest.mark.parametrize("storage", [nflow.F16, nflow.BF16])
+def test_server_tau_floor_keeps_low_tau_requests_in_fp32(storage):
+    """Below the tau floor a request is stored in FP32 (its own batch) and its
+    result equals the FP32 solo run; the others stay in the server's 16-bit
+    storage and equal their 16-bit solo runs, both bitwise."""
+    inputs = mixed_inputs(storage)
+    steps = [19, 23, 31, 7, 11]
+    srv = nflow.Server(budget_bytes=512 << 20, storage=storage, max_resident=4, steps_per_tick=6)
+    srv.set_tau_floor(0.57)
+    for i, (inp, n) in enumerate(zip(inputs, steps)):
+        srv.submit(inp, n, tag=200 + i, want_fields=True)
+    srv.drain()
+    res = {r.tag: r for r in srv.poll(16)}
+    low = [i for i, inp in enumerate(inputs) if inp.tau < 0.57]
+    assert low and len(low) < len(inputs)
+    for i, (inp, n) in enumerate(zip(inputs, steps)):
+        assert res[200 + i].status == 0
+        solo_in = nflow.defaults(**{k: getattr(inp, k) for k, _ in inp._fields_ if k != "ref_vel"},
+                                 ref_vel=tuple(inp.ref_vel))
+        solo_in.storage = nflow.F32 if i in low else storage
+        s = nflow.Solver(solo_in)
+        s.initialize()
+        s.run(n)
+        want = s.read_macro()
+        s.close()
+        got = srv.fetch_macro(200 + i, (inp.ny, inp.nz, inp.nx))
+        for a, w in zip(got, want):
+            assert np.array_equal(a, w), (i, i in low)
+    srv.close()
+    with pytest.raises(nflow.NflowError):
+        nflow.Server(budget_bytes=64 << 20, storage=storage).set_tau_floor(-1.0)
