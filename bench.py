@@ -34,6 +34,9 @@ WORKLOADS = {
     "cavity256_f16": dict(nx=256, ny=256, nz=256, casetype=1, storage=1,
                           desc="3D lid-driven cavity 256^3, D3Q19 MRT + LES (Cs 0.17), tau 0.6, FP16 storage "
                                "of f - w_q, FP32 arithmetic"),
+    "cavity256_f16sr": dict(nx=256, ny=256, nz=256, casetype=1, storage=1, sr=1,
+                            desc="3D lid-driven cavity 256^3, D3Q19 MRT + LES (Cs 0.17), tau 0.6, FP16 storage "
+                                 "of f - w_q with stochastic rounding, FP32 arithmetic"),
     "cavity256_bf16": dict(nx=256, ny=256, nz=256, casetype=1, storage=2,
                            desc="3D lid-driven cavity 256^3, D3Q19 MRT + LES (Cs 0.17), tau 0.6, BF16 storage "
                                 "of f - w_q, FP32 arithmetic"),
@@ -53,9 +56,11 @@ BYTES_PER_UPDATE = {0: 152, 1: 76, 2: 76}
 
 def make_input(nflow, w):
     tau = 0.56 if w["casetype"] == 0 else 0.6
-    return nflow.defaults(nx=w["nx"], ny=w["ny"], nz=w["nz"], casetype=w["casetype"], storage=w["storage"],
-                          enable_mrt=1, enable_les=1, smagorinsky=0.17, tau=tau, ref_vel=(0.05, 0.0, 0.0),
-                          init_lid_velocity=0, inlet_peak=0.05, perturb=0.01, seed=2024)
+    p = nflow.defaults(nx=w["nx"], ny=w["ny"], nz=w["nz"], casetype=w["casetype"], storage=w["storage"],
+                       enable_mrt=1, enable_les=1, smagorinsky=0.17, tau=tau, ref_vel=(0.05, 0.0, 0.0),
+                       init_lid_velocity=0, inlet_peak=0.05, perturb=0.01, seed=2024)
+    p.stochastic_rounding = w.get("sr", 0)
+    return p
 
 
 def traffic_from_profiles(name):
@@ -445,7 +450,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": DTYPE[w["storage"]],
+        "dtype": DTYPE[w["storage"]] + (", stochastic rounding" if w.get("sr") else ""),
         "data": "synthetic (lattice initialised at rest, rho = 1)",
         "config": {"workload": w["desc"], "nx": w["nx"], "ny": w["ny"], "nz": w["nz"], "nodes": n,
                    "tuning": b.get_tuning(), "device": name, "cus": cus},

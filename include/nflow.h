@@ -82,6 +82,12 @@ typedef struct nflow_input {
   /* pipe: nodes with (y - cy)^2 + (z - cz)^2 > r^2 are solid (kernel.cu:1090
    * uses cy = cz = 25, r = 20); the inlet velocity is ref_vel (kernel.cu:601) */
   double pipe_radius, pipe_cy, pipe_cz;
+  /* FP16/BF16 storage: 1 rounds each stored f - w_q stochastically (a hash of
+   * the node's coordinates and the value's own bits picks the direction, with
+   * probability by distance) instead of to nearest, so that rounding errors
+   * do not correlate from step to step and accumulate over long runs;
+   * deterministic, ignored for FP32.  0: round to nearest (the default). */
+  int stochastic_rounding;
 } nflow_input;
 
 typedef struct nflow_solver nflow_solver;

@@ -58,6 +58,7 @@ class Batch {
   int grid = 0;
   bool tables_dirty = true;
   bool masked = false;  // some lattice has solid nodes: step with the masked kernel
+  bool sr_any = false;  // some lattice rounds its 16-bit stores stochastically
 
   int gparity = 0;  // parity argument of the next step launch
   hipGraphExec_t graph[2] = {nullptr, nullptr};

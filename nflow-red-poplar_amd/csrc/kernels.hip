@@ -148,6 +148,9 @@ struct StF16 {
   using T = _Float16;
   static constexpr bool shifted = true;
   static constexpr bool buffer = true;
+  static constexpr uint32_t mant = 10;              // stored mantissa bits
+  static constexpr uint32_t min_exp = 0x38800000u;  // float bits of 2^-14, the smallest normal: below it
+                                                    // the spacing stays 2^-24 (subnormals)
   __device__ static inline float dec(T v) { return (float)v; }
   __device__ static inline T enc(float v) { return (_Float16)v; }
 };
@@ -158,6 +161,10 @@ struct StBF16 {
   using T = __bf16;
   static constexpr bool shifted = true;
   static constexpr bool buffer = true;
+  static constexpr uint32_t mant = 7;
+  // floor of the exponent used for the rounding spacing: 2^-119, so that the
+  // spacing 2^-126 stays a normal float (values below it round as zero does)
+  static constexpr uint32_t min_exp = 0x04000000u;
   __device__ static inline float dec(T v) { return (float)v; }
   __device__ static inline T enc(float v) { return (__bf16)v; }
 };
@@ -165,6 +172,22 @@ struct StBF16 {
 template <class St>
 __device__ constexpr float shift_of(int q) {
   return St::shifted ? W[q] : 0.0f;
+}
+
+// Stochastic rounding to a 16-bit format: v plus a uniform dither in
+// (-1/2, 1/2) of the format's spacing at v, then round to nearest, rounds up
+// with probability equal to v's distance from the grid point below.  Eight
+// dither bits come from a 24-bit multiplicative hash of the value's bits and
+// a per-node, per-direction key, so the same inputs always give the same
+// result (batched and solo runs stay bitwise equal).
+template <class St>
+__device__ inline typename St::T enc_stochastic(float v, uint32_t key) {
+  const uint32_t b = __float_as_uint(v);
+  const uint32_t h = __umul24(b ^ key, 0x9E3779u);
+  const float d = (float)((h >> 8) & 0xFFu) + 0.5f;  // 0.5 .. 255.5
+  const uint32_t eb = max(b & 0x7F800000u, St::min_exp);
+  const float ulp = __uint_as_float(eb - (St::mant << 23));  // spacing of the 16-bit grid at v
+  return (typename St::T)fmaf(d, ulp * (1.0f / 256.0f), fmaf(-0.5f, ulp, v));
 }
 
 // ---------------------------------------------------------------------------
@@ -526,7 +549,9 @@ __device__ inline void collide_mrt(float f[Q], float rho, float ir, float jx, fl
 }
 
 // Macroscopic -> side planes -> LES -> collision -> unit-stride store.
-template <class St, class Acc>
+// SR: the instantiation carries the stochastic-rounding store (taken for
+// lattices with LatticeDev::sr set); without it the code is not generated.
+template <class St, bool SR, class Acc>
 __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], const Acc& out, float* __restrict__ side_out,
                                        uint32_t nr, int x, int y, int z) {
   float rho, jx, jy, jz;
@@ -545,6 +570,16 @@ __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], const Ac
     collide_srt(f, rho, ir, jx, jy, jz, hw_rcp(L.tau + te));
   }
   const uint32_t ps = (uint32_t)L.pstride;
+  if constexpr (St::shifted && SR) {
+    if (L.sr) {
+      const uint32_t key = ((uint32_t)x * 0x9E3779B1u) ^ ((uint32_t)y * 0x85EBCA77u) ^ ((uint32_t)z * 0xC2B2AE3Du);
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        out.put(pidx<typename St::T>(q, nr, ps),
+                enc_stochastic<St>(f[q] - shift_of<St>(q), key ^ ((uint32_t)q * 0x27D4EB2Fu)));
+      return;
+    }
+  }
 #pragma unroll
   for (int q = 0; q < Q; ++q) out.put(pidx<typename St::T>(q, nr, ps), St::enc(f[q] - shift_of<St>(q)));
 }
@@ -569,7 +604,7 @@ __device__ inline void coords(const LatticeDev& L, uint32_t n, int& x, int& y, i
 
 // One node of a work item: pull + boundary rules, collide, store.  `in` and
 // `out` address the arrays from node bn on (0: the whole array).
-template <class St, bool MASK, class Acc>
+template <class St, bool MASK, bool SR, class Acc>
 __device__ __forceinline__ void step_node(const LatticeDev& L, const WorkItem& wi, int p, uint32_t bn, const Acc& in,
                                  const Acc& out) {
   if (threadIdx.x >= wi.count) return;
@@ -579,7 +614,7 @@ __device__ __forceinline__ void step_node(const LatticeDev& L, const WorkItem& w
   coords(L, n, x, y, z);
   float f[Q];
   gather<St, MASK>(L, in, L.side[p], n, n - bn, x, y, z, f);
-  relax_and_store<St>(L, f, out, L.side[p ^ 1], n - bn, x, y, z);
+  relax_and_store<St, SR>(L, f, out, L.side[p ^ 1], n - bn, x, y, z);
 }
 
 // MASK = true: at least one lattice of the launch has solid nodes (the plain
@@ -591,7 +626,7 @@ __device__ __forceinline__ void step_node(const LatticeDev& L, const WorkItem& w
 // node of the item pulls from, and its lanes index the window with n - bn.
 // pidx(q, n) - pidx(q, n - bn) = bn * Q for bn a multiple of B, so the
 // element addressed is the same as with whole-array indexing.
-template <class St, int BS, bool MASK>
+template <class St, int BS, bool MASK, bool SR>
 __global__ void __launch_bounds__(BS) NFLOW_STEP_ATTR k_step(const LatticeDev* __restrict__ lats, const WorkItem* __restrict__ items,
                                              int nitems, int parity) {
   using T = typename St::T;
@@ -605,10 +640,10 @@ __global__ void __launch_bounds__(BS) NFLOW_STEP_ATTR k_step(const LatticeDev* _
       const uint32_t bn = (wi.start > reach ? wi.start - reach : 0u) & ~(B - 1);
       const uint64_t skip = (uint64_t)bn * Q;  // elements before the window
       const uint64_t bytes = ((uint64_t)L.pstride * Q - skip) * sizeof(T);
-      step_node<St, MASK>(L, wi, p, bn, BPops<T>(static_cast<T*>(L.buf[p]) + skip, bytes),
+      step_node<St, MASK, SR>(L, wi, p, bn, BPops<T>(static_cast<T*>(L.buf[p]) + skip, bytes),
                           BPops<T>(static_cast<T*>(L.buf[p ^ 1]) + skip, bytes));
     } else {
-      step_node<St, MASK>(L, wi, p, 0u, GPops<T>{static_cast<T*>(L.buf[p])}, GPops<T>{static_cast<T*>(L.buf[p ^ 1])});
+      step_node<St, MASK, SR>(L, wi, p, 0u, GPops<T>{static_cast<T*>(L.buf[p])}, GPops<T>{static_cast<T*>(L.buf[p ^ 1])});
     }
   }
 }
@@ -666,7 +701,8 @@ __global__ void __launch_bounds__(256) k_init(LatticeDev L, int cur, const float
   const uint32_t ps = (uint32_t)L.pstride;
 #pragma unroll
   for (int q = 0; q < Q; ++q) streamed.put(pidx<typename St::T>(q, n, ps), St::enc(f[q] - shift_of<St>(q)));
-  relax_and_store<St>(L, f, GPops<typename St::T>{static_cast<typename St::T*>(L.buf[cur])}, L.side[cur], n, x, y, z);
+  relax_and_store<St, true>(L, f, GPops<typename St::T>{static_cast<typename St::T*>(L.buf[cur])}, L.side[cur], n, x, y,
+                            z);
 }
 
 // ---------------------------------------------------------------------------
@@ -844,41 +880,45 @@ __global__ void __launch_bounds__(256) k_fma_chains(float* __restrict__ out, flo
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <class St, bool MASK>
+template <class St, bool MASK, bool SR>
 static hipError_t step_bs(int block, int grid, hipStream_t st, const LatticeDev* d_lats, const WorkItem* d_items,
                           int nitems, int parity) {
   switch (block) {
     case 64:
-      hipLaunchKernelGGL((k_step<St, 64, MASK>), dim3(grid), dim3(64), 0, st, d_lats, d_items, nitems, parity);
+      hipLaunchKernelGGL((k_step<St, 64, MASK, SR>), dim3(grid), dim3(64), 0, st, d_lats, d_items, nitems, parity);
       break;
     case 128:
-      hipLaunchKernelGGL((k_step<St, 128, MASK>), dim3(grid), dim3(128), 0, st, d_lats, d_items, nitems, parity);
+      hipLaunchKernelGGL((k_step<St, 128, MASK, SR>), dim3(grid), dim3(128), 0, st, d_lats, d_items, nitems, parity);
       break;
     case 256:
-      hipLaunchKernelGGL((k_step<St, 256, MASK>), dim3(grid), dim3(256), 0, st, d_lats, d_items, nitems, parity);
+      hipLaunchKernelGGL((k_step<St, 256, MASK, SR>), dim3(grid), dim3(256), 0, st, d_lats, d_items, nitems, parity);
       break;
     case 512:
-      hipLaunchKernelGGL((k_step<St, 512, MASK>), dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity);
+      hipLaunchKernelGGL((k_step<St, 512, MASK, SR>), dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity);
       break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <class St>
+template <class St, bool SR>
 static hipError_t step_mask(bool masked, int block, int grid, hipStream_t st, const LatticeDev* d_lats,
                             const WorkItem* d_items, int nitems, int parity) {
-  return masked ? step_bs<St, true>(block, grid, st, d_lats, d_items, nitems, parity)
-                : step_bs<St, false>(block, grid, st, d_lats, d_items, nitems, parity);
+  return masked ? step_bs<St, true, SR>(block, grid, st, d_lats, d_items, nitems, parity)
+                : step_bs<St, false, SR>(block, grid, st, d_lats, d_items, nitems, parity);
 }
 
-hipError_t launch_step(int storage, bool masked, int block, int grid, hipStream_t st, const LatticeDev* d_lats,
-                       const WorkItem* d_items, int nitems, int parity) {
+hipError_t launch_step(int storage, bool masked, bool sr, int block, int grid, hipStream_t st,
+                       const LatticeDev* d_lats, const WorkItem* d_items, int nitems, int parity) {
   if (grid <= 0 || nitems <= 0) return hipSuccess;
   switch (storage) {
-    case ST_F32: return step_mask<StF32>(masked, block, grid, st, d_lats, d_items, nitems, parity);
-    case ST_F16: return step_mask<StF16>(masked, block, grid, st, d_lats, d_items, nitems, parity);
-    case ST_BF16: return step_mask<StBF16>(masked, block, grid, st, d_lats, d_items, nitems, parity);
+    case ST_F32: return step_mask<StF32, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
+    case ST_F16:
+      return sr ? step_mask<StF16, true>(masked, block, grid, st, d_lats, d_items, nitems, parity)
+                : step_mask<StF16, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
+    case ST_BF16:
+      return sr ? step_mask<StBF16, true>(masked, block, grid, st, d_lats, d_items, nitems, parity)
+                : step_mask<StBF16, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
   }
   return hipErrorInvalidValue;
 }

@@ -62,7 +62,7 @@ struct alignas(16) LatticeDev {
   uint32_t n;         // interior nodes
   FastDiv div_nx, div_nxz;
   int casetype, model, les, parity0;
-  int init_lid, pad0, pad1, pad2;
+  int init_lid, sr, pad1, pad2;  // sr: stochastic rounding of 16-bit stores
   float tau, les_c, refden, inv_refden;
   float weps, wepsj, wxx, pad3;
   float ulid[4];
@@ -78,9 +78,10 @@ struct alignas(16) WorkItem {
 
 // Kernel launchers (kernels.hip).
 // masked: use the instantiation that honours LatticeDev::mask (any lattice of
-// the launch has solid nodes).
-hipError_t launch_step(int storage, bool masked, int block, int grid, hipStream_t st, const LatticeDev* d_lats,
-                       const WorkItem* d_items, int nitems, int parity);
+// the launch has solid nodes); sr: the one that can round stochastically (any
+// lattice of the launch has LatticeDev::sr set).
+hipError_t launch_step(int storage, bool masked, bool sr, int block, int grid, hipStream_t st,
+                       const LatticeDev* d_lats, const WorkItem* d_items, int nitems, int parity);
 hipError_t launch_init(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src_aos);
 hipError_t launch_export(int storage, hipStream_t st, const LatticeDev& L, int prev, int prev_streamed, float* aos,
                          float* rho, float* ux, float* uy, float* uz, double* partials, int* nblocks_out);

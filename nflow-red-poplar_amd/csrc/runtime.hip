@@ -98,6 +98,7 @@ void lattice_layout(HostLattice& h, const nflow_input& in, int storage, void* me
   d.les = (in.enable_les && in.smagorinsky > 0.0) ? 1 : 0;
   d.parity0 = 0;
   d.init_lid = in.init_lid_velocity ? 1 : 0;
+  d.sr = (in.stochastic_rounding && storage != NFLOW_STORAGE_F32) ? 1 : 0;
   d.tau = (float)in.tau;
   d.les_c = (float)(18.0 * 1.41421356237 * in.smagorinsky * in.smagorinsky);
   d.refden = (float)in.ref_den;
@@ -262,10 +263,12 @@ int Batch::upload_tables() {
   std::vector<WorkItem> items;
   std::vector<LatticeDev> devs(lats.size());
   masked = false;
+  sr_any = false;
   for (size_t li = 0; li < lats.size(); ++li) {
     const HostLattice& h = lats[li];
     devs[li] = h.dev;
     if (h.dev.mask) masked = true;
+    if (h.dev.sr) sr_any = true;
     const uint32_t nx = (uint32_t)h.in.nx, nxz = (uint32_t)h.in.nx * (uint32_t)h.in.nz;
     auto emit = [&](uint32_t a, uint32_t b) {
       for (uint32_t s = a; s < b; s += (uint32_t)bs)
@@ -345,7 +348,7 @@ int Batch::build_graph(int parity) {
   int p = parity;
   hipError_t err = hipSuccess;
   for (int s = 0; s < tuning.graph_steps && err == hipSuccess; ++s) {
-    err = launch_step(storage, masked, tuning.block, grid, stream, d_lats, d_items, nitems, p);
+    err = launch_step(storage, masked, sr_any, tuning.block, grid, stream, d_lats, d_items, nitems, p);
     p ^= 1;
   }
   hipError_t e2 = hipStreamEndCapture(stream, &g);
@@ -380,7 +383,7 @@ int Batch::launch_steps(int nsteps) {
     launches += (long long)reps * gs;
   }
   for (int s = 0; s < left; ++s) {
-    NFLOW_HIP(launch_step(storage, masked, tuning.block, grid, stream, d_lats, d_items, nitems, gparity));
+    NFLOW_HIP(launch_step(storage, masked, sr_any, tuning.block, grid, stream, d_lats, d_items, nitems, gparity));
     gparity ^= 1;
     ++launches;
   }

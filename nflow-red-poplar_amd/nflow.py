@@ -36,6 +36,7 @@ class Input(ctypes.Structure):
         ("w_eps", ctypes.c_double), ("w_epsj", ctypes.c_double), ("w_xx", ctypes.c_double),
         ("inlet_peak", ctypes.c_double), ("perturb", ctypes.c_double), ("seed", ctypes.c_ulonglong),
         ("pipe_radius", ctypes.c_double), ("pipe_cy", ctypes.c_double), ("pipe_cz", ctypes.c_double),
+        ("stochastic_rounding", ctypes.c_int),
     ]
 
 
