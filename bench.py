@@ -207,6 +207,8 @@ def bench_serve(args):
     reqs = workloads.request_stream(seed=args.seed, count=args.requests, mean_gap_s=0.05,
                                     channel_fraction=args.channel_fraction, step_scale=args.step_scale,
                                     storage=nflow.F16)
+    for r in reqs:
+        r["input"].stochastic_rounding = 1 if args.stochastic_rounding else 0
     _, _, hbm = nflow.device_info()
     budget = min(200 << 30, int(hbm * 0.75))
     srv = nflow.Server(budget, storage=nflow.F16, max_resident=64, steps_per_tick=args.tick)
@@ -285,7 +287,7 @@ def bench_serve(args):
         "peak_resident": st["peak_resident"], "peak_bytes": st["peak_bytes"],
         "server_busy_s": round(st["busy_s"], 3), "server_step_s": round(st["step_s"], 3), "ticks": st["ticks"],
         "step_mlups": round(st["node_updates"] / max(st["step_s"], 1e-9) / 1e6, 1),
-        "tau_floor": args.tau_floor,
+        "tau_floor": args.tau_floor, "stochastic_rounding": bool(args.stochastic_rounding),
         "fp32_requests": sum(1 for r in reqs if r["input"].tau < args.tau_floor),
         "rel_l2_u_vs_fp32_solo": errs,
         "max_rel_l2_u_checked": max(e["rel_l2_u"] for e in errs),
@@ -380,6 +382,8 @@ def main():
     ap.add_argument("--step-scale", type=float, default=0.1, help="serve mode: scale of the per-request steps")
     ap.add_argument("--tick", type=int, default=100, help="serve mode: steps per tick")
     ap.add_argument("--channel-fraction", type=float, default=0.5, help="serve mode: share of channel requests")
+    ap.add_argument("--stochastic-rounding", action="store_true",
+                    help="serve mode: round the 16-bit stores of every request stochastically")
     ap.add_argument("--tau-floor", type=float, default=0.0,
                     help="serve mode: requests with tau below this are kept in FP32 storage")
     args = ap.parse_args()
