@@ -23,30 +23,31 @@ sys.path.insert(0, os.path.join(ROOT, "nflow-red-poplar_amd"))
 
 METRIC = "D3Q19 MRT-LES MLUPS & % HBM peak; served-batch p50/p99 latency at ≤1e-3 err"
 HBM_PEAK_GBS = 8000.0
+CAVITY, CHANNEL = 1, 2  # nflow_input.casetype (include/nflow.h)
 
 WORKLOADS = {
-    "cavity256": dict(nx=256, ny=256, nz=256, casetype=1, storage=0,
+    "cavity256": dict(nx=256, ny=256, nz=256, casetype=CAVITY, storage=0,
                       desc="3D lid-driven cavity 256^3, D3Q19 MRT + Smagorinsky LES (Cs 0.17), tau 0.6, "
                            "lid u_x 0.05, FP32 populations"),
-    "cavity128": dict(nx=128, ny=128, nz=128, casetype=1, storage=0,
+    "cavity128": dict(nx=128, ny=128, nz=128, casetype=CAVITY, storage=0,
                       desc="3D lid-driven cavity 128^3, D3Q19 MRT + Smagorinsky LES (Cs 0.17), tau 0.6, "
                            "lid u_x 0.05, FP32 populations"),
-    "cavity256_f16": dict(nx=256, ny=256, nz=256, casetype=1, storage=1,
+    "cavity256_f16": dict(nx=256, ny=256, nz=256, casetype=CAVITY, storage=1,
                           desc="3D lid-driven cavity 256^3, D3Q19 MRT + LES (Cs 0.17), tau 0.6, FP16 storage "
                                "of f - w_q, FP32 arithmetic"),
-    "cavity256_f16sr": dict(nx=256, ny=256, nz=256, casetype=1, storage=1, sr=1,
+    "cavity256_f16sr": dict(nx=256, ny=256, nz=256, casetype=CAVITY, storage=1, sr=1,
                             desc="3D lid-driven cavity 256^3, D3Q19 MRT + LES (Cs 0.17), tau 0.6, FP16 storage "
                                  "of f - w_q with stochastic rounding, FP32 arithmetic"),
-    "cavity256_bf16": dict(nx=256, ny=256, nz=256, casetype=1, storage=2,
+    "cavity256_bf16": dict(nx=256, ny=256, nz=256, casetype=CAVITY, storage=2,
                            desc="3D lid-driven cavity 256^3, D3Q19 MRT + LES (Cs 0.17), tau 0.6, BF16 storage "
                                 "of f - w_q, FP32 arithmetic"),
-    "slab256x32": dict(nx=256, ny=32, nz=256, casetype=1, storage=0,
+    "slab256x32": dict(nx=256, ny=32, nz=256, casetype=CAVITY, storage=0,
                        desc="one GPU's share of a 256^3 cavity cut into 8 y-slabs: 256x32x256, D3Q19 MRT + LES "
                             "(Cs 0.17), tau 0.6, FP32 populations (slab faces run the wall rules)"),
-    "slab256x32_f16": dict(nx=256, ny=32, nz=256, casetype=1, storage=1,
+    "slab256x32_f16": dict(nx=256, ny=32, nz=256, casetype=CAVITY, storage=1,
                            desc="one GPU's share of a 256^3 cavity cut into 8 y-slabs: 256x32x256, D3Q19 MRT + "
                                 "LES (Cs 0.17), tau 0.6, FP16 storage of f - w_q (slab faces run the wall rules)"),
-    "channel": dict(nx=512, ny=128, nz=128, casetype=0, storage=0,
+    "channel": dict(nx=512, ny=128, nz=128, casetype=CHANNEL, storage=0,
                     desc="square-duct channel 512x128x128, D3Q19 MRT + LES (Cs 0.17), tau 0.56, parabolic "
                          "inlet peak 0.05 with seeded 1% perturbation, pressure outlet, FP32 populations"),
 }
@@ -55,7 +56,7 @@ BYTES_PER_UPDATE = {0: 152, 1: 76, 2: 76}
 
 
 def make_input(nflow, w):
-    tau = 0.56 if w["casetype"] == 0 else 0.6
+    tau = 0.56 if w["casetype"] == CHANNEL else 0.6
     p = nflow.defaults(nx=w["nx"], ny=w["ny"], nz=w["nz"], casetype=w["casetype"], storage=w["storage"],
                        enable_mrt=1, enable_les=1, smagorinsky=0.17, tau=tau, ref_vel=(0.05, 0.0, 0.0),
                        init_lid_velocity=0, inlet_peak=0.05, perturb=0.01, seed=2024)
@@ -103,7 +104,7 @@ def cpu_baseline(w, budget_s=15.0, single_budget_s=8.0):
     import oracle
 
     p = oracle.default_params(nx=w["nx"], ny=w["ny"], nz=w["nz"], casetype=w["casetype"], enable_mrt=1,
-                              enable_les=1, smagorinsky=0.17, tau=0.56 if w["casetype"] == 0 else 0.6,
+                              enable_les=1, smagorinsky=0.17, tau=0.56 if w["casetype"] == CHANNEL else 0.6,
                               ref_vel=(0.05, 0.0, 0.0), init_lid_velocity=0, inlet_peak=0.05, perturb=0.01,
                               seed=2024)
     L = oracle.Lattice(p)
@@ -310,12 +311,12 @@ def bench_precision(args):
     want = args.cases.split(",")
     if "channel" in want:
         cases.append(("channel 512x128x128 MRT-LES Cs 0.17 tau 0.56, inlet peak 0.05 + 1% seeded", args.channel_steps,
-                      dict(nx=512, ny=128, nz=128, casetype=0, enable_mrt=1, enable_les=1, smagorinsky=0.17,
+                      dict(nx=512, ny=128, nz=128, casetype=CHANNEL, enable_mrt=1, enable_les=1, smagorinsky=0.17,
                            tau=0.56, inlet_peak=0.05, perturb=0.01, seed=2024, ref_vel=(0.05, 0.0, 0.0),
                            init_lid_velocity=0)))
     if "cavity" in want:
         cases.append((f"cavity 128^3 MRT-LES Cs 0.17 tau {args.cavity_tau:g}, lid 0.05", args.cavity_steps,
-                      dict(nx=128, ny=128, nz=128, casetype=1, enable_mrt=1, enable_les=1, smagorinsky=0.17,
+                      dict(nx=128, ny=128, nz=128, casetype=CAVITY, enable_mrt=1, enable_les=1, smagorinsky=0.17,
                            tau=args.cavity_tau, ref_vel=(0.05, 0.0, 0.0), init_lid_velocity=0)))
 
     def oracle_macro(c, steps, fp32):

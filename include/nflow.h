@@ -36,9 +36,11 @@ extern "C" {
 #define NFLOW_ZDIM 1
 #define NFLOW_YDIM 2
 
+/* Case numbers pass the reference's values through unchanged (common.h:26-27):
+ * a caller may copy Inputval.casetype straight into nflow_input.casetype. */
+#define NFLOW_PIPE 0    /* common.h:27 POISEUILLE: circular pipe of solid nodes (kernel.cu:1078-1095) */
 #define NFLOW_CAVITY 1  /* common.h:26 */
-#define NFLOW_CHANNEL 0 /* square duct: parabolic velocity inlet, pressure outlet, walls */
-#define NFLOW_PIPE 2    /* common.h:27 POISEUILLE: circular pipe of solid nodes (kernel.cu:1078-1095) */
+#define NFLOW_CHANNEL 2 /* no reference value: square duct, parabolic velocity inlet, pressure outlet, walls */
 
 #define NFLOW_NODE_FLUID (1 << 1) /* common.h:32-37 */
 #define NFLOW_NODE_FIXEDWALL (1 << 2)
@@ -66,7 +68,7 @@ typedef struct nflow_input {
   int nx, ny, nz;        /* interior nodes per axis */
   int maxitr;            /* steps the driver runs after initialisation */
   int nwrite;            /* output period of the driver (kernel.cu:205) */
-  int casetype;          /* NFLOW_CAVITY or NFLOW_CHANNEL */
+  int casetype;          /* NFLOW_PIPE (0), NFLOW_CAVITY (1) or NFLOW_CHANNEL (2) */
   int enable_mrt;        /* 1: MRT collision, 0: SRT/BGK */
   int enable_les;        /* Smagorinsky eddy viscosity */
   int storage;           /* NFLOW_STORAGE_* */

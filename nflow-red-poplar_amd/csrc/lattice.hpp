@@ -6,7 +6,7 @@
 
 namespace nflow {
 
-enum : int { CASE_CHANNEL = 0, CASE_CAVITY = 1, CASE_PIPE = 2 };
+enum : int { CASE_PIPE = 0, CASE_CAVITY = 1, CASE_CHANNEL = 2 };  // NFLOW_PIPE/CAVITY/CHANNEL
 enum : int { MODEL_SRT = 0, MODEL_MRT = 1 };
 enum : int { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2 };
 enum : int {

@@ -222,6 +222,10 @@ void orc_inlet_profile(const orc_params* p, double* uin) {
  * Single-cell maths, in the working precision `real`.
  * ------------------------------------------------------------------------ */
 static real R_DIR[3][ORC_Q], R_W[ORC_Q], R_M[ORC_Q][ORC_Q], R_MINV[ORC_Q][ORC_Q];
+/* the same two matrices stored column by column, [k][i] (padded to 20 rows),
+ * so that the mat-vecs below can run over i in vector registers */
+#define ORC_QP 20
+static real R_MT[ORC_Q][ORC_QP], R_MINVT[ORC_Q][ORC_QP];
 
 __attribute__((constructor)) static void orc_init_real_tables(void) {
   orc_init_tables();
@@ -231,6 +235,8 @@ __attribute__((constructor)) static void orc_init_real_tables(void) {
     for (int k = 0; k < ORC_Q; ++k) {
       R_M[q][k] = (real)g_M[q][k];
       R_MINV[q][k] = (real)g_Minv[q][k];
+      R_MT[k][q] = R_M[q][k];
+      R_MINVT[k][q] = R_MINV[q][k];
     }
   }
 }
@@ -238,8 +244,11 @@ __attribute__((constructor)) static void orc_init_real_tables(void) {
 /* feq, lbm_codelets.cpp:222-231 */
 static void cell_feq(real rho, const real u[3], real feq[ORC_Q]) {
   const real usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  /* e_q from the integer tables (the values of R_DIR), unrolled so that the
+   * compiler sees the 0 / +-1 factors */
+#pragma GCC unroll 19
   for (int q = 0; q < ORC_Q; ++q) {
-    const real eu = R_DIR[ORC_XDIM][q] * u[0] + R_DIR[ORC_YDIM][q] * u[1] + R_DIR[ORC_ZDIM][q] * u[2];
+    const real eu = (real)EX[q] * u[0] + (real)EY[q] * u[1] + (real)EZ[q] * u[2];
     feq[q] = R_W[q] * rho * ((real)1 + (real)3 * eu + (real)4.5 * eu * eu - (real)1.5 * usq);
   }
 }
@@ -249,8 +258,9 @@ static real cell_tau_eddy(const orc_params* p, const real f[ORC_Q], real rho, co
   real feq[ORC_Q];
   cell_feq(rho, u, feq);
   real phi[3][3] = {{0}};
+#pragma GCC unroll 19
   for (int q = 0; q < ORC_Q; ++q) {
-    const real e[3] = {R_DIR[ORC_XDIM][q], R_DIR[ORC_YDIM][q], R_DIR[ORC_ZDIM][q]};
+    const real e[3] = {(real)EX[q], (real)EY[q], (real)EZ[q]}; /* = R_DIR[XDIM/YDIM/ZDIM][q] */
     const real mul = f[q] - feq[q];
     for (int a = 0; a < 3; ++a)
       for (int b = 0; b < 3; ++b) phi[a][b] += e[a] * (e[b] * mul);
@@ -288,21 +298,19 @@ static void cell_mrt(const orc_params* p, const real f[ORC_Q], real rho, const r
   meq[15] = j[2] * j[0] / r0;
   meq[16] = meq[17] = meq[18] = 0;
   const real snu = (real)1.0 / ((real)p->tau + tau_eddy);
-  real m1[ORC_Q], tmp[ORC_Q];
-  for (int i = 0; i < ORC_Q; ++i) {
-    real s = 0;
-    for (int k = 0; k < ORC_Q; ++k) s += R_M[i][k] * f[k];
-    m1[i] = s;
-  }
+  /* m1 = M f and s = M^-1 tmp as dense 19 x 19 products; each row sums its
+   * 19 terms in the order k = 0..18, as the row loops of the reference do,
+   * with the loop over rows innermost (the same arithmetic, vectorisable) */
+  real m1[ORC_QP] = {0}, tmp[ORC_Q], s[ORC_QP] = {0};
+  for (int k = 0; k < ORC_Q; ++k)
+    for (int i = 0; i < ORC_QP; ++i) m1[i] += R_MT[k][i] * f[k];
   for (int i = 0; i < ORC_Q; ++i) {
     const real s_i = p->s_rates[i] < 0.0 ? snu : (real)p->s_rates[i];
     tmp[i] = s_i * (m1[i] - meq[i]);
   }
-  for (int i = 0; i < ORC_Q; ++i) {
-    real s = 0;
-    for (int k = 0; k < ORC_Q; ++k) s += R_MINV[i][k] * tmp[k];
-    fout[i] = f[i] - s;
-  }
+  for (int k = 0; k < ORC_Q; ++k)
+    for (int i = 0; i < ORC_QP; ++i) s[i] += R_MINVT[k][i] * tmp[k];
+  for (int i = 0; i < ORC_Q; ++i) fout[i] = f[i] - s[i];
 }
 
 /* keCollision_SRT body, kernel.cu:466-486 */
@@ -568,6 +576,16 @@ static void phase_stream(orc_lattice* L) {
     if (orc_slice_chunk(t, (const int(*)[3][2])L->slice, c)) continue;
     const int nx2 = c[6] + 2, nz2 = c[8] + 2, nslave2 = nx2 * nz2;
     const int* mv = L->move[t];
+    /* loc = x + z nx2 + y nslave2 + Move[q] is the donor's index inside the
+     * tile's padded slice; Move[q] = ox + oz nx2 + oy nslave2 with offsets in
+     * {0, 1, 2}, so the donor's padded coordinates are (x + ox, y + oy, z + oz)
+     * (decoded once per tile instead of once per link) */
+    int ox[ORC_Q], oy[ORC_Q], oz[ORC_Q];
+    for (int q = 0; q < ORC_Q; ++q) {
+      oy[q] = mv[q] / nslave2;
+      oz[q] = (mv[q] % nslave2) / nx2;
+      ox[q] = mv[q] % nx2;
+    }
 #pragma omp parallel for
     for (int y = 0; y < c[7]; ++y)
       for (int z = 0; z < c[8]; ++z)
@@ -577,8 +595,7 @@ static void phase_stream(orc_lattice* L) {
           const int ds = L->dirstate[n];
           for (int q = 0; q < ORC_Q; ++q) {
             if (!((ds >> q) & 1)) continue;
-            const int loc = x + z * nx2 + y * nslave2 + mv[q]; /* index inside the tile's padded slice */
-            const int ly = loc / nslave2, rem = loc % nslave2, lz = rem / nx2, lx = rem % nx2;
+            const int lx = x + ox[q], ly = y + oy[q], lz = z + oz[q];
             const size_t g = pid(L, c[0] + lx, c[2] + ly, c[4] + lz);
             if (L->mask && L->mask[nid(L, c[0] + lx - 1, c[2] + ly - 1, c[4] + lz - 1)]) {
               L->arF[n * ORC_Q + OPP[q]] = L->arFTemp[pid(L, x + c[0] + 1, y + c[2] + 1, z + c[4] + 1) * ORC_Q + q];

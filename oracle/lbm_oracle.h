@@ -39,9 +39,9 @@ extern "C" {
 #define ORC_NODE_OUTLET (1 << 5)
 #define ORC_NODE_STRUCT (1 << 6)
 
-#define ORC_CAVITY 1 /* common.h:26-27; 0 is the square-duct channel here */
-#define ORC_CHANNEL 0
-#define ORC_PIPE 2 /* common.h:27 POISEUILLE of the CUDA path: the duct with a circular solid mask */
+#define ORC_PIPE 0   /* common.h:27 POISEUILLE of the CUDA path: the duct with a circular solid mask */
+#define ORC_CAVITY 1 /* common.h:26 */
+#define ORC_CHANNEL 2 /* the square-duct channel (no reference value) */
 
 typedef struct orc_params {
   int nx, ny, nz;

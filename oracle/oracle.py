@@ -11,7 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 Q = 19
-CAVITY, CHANNEL, PIPE = 1, 0, 2
+PIPE, CAVITY, CHANNEL = 0, 1, 2  # POISEUILLE = 0 and CAVITY = 1 as in common.h:26-27
 NODE_FLUID, NODE_FIXEDWALL, NODE_MOVINGWALL = 1 << 1, 1 << 2, 1 << 3
 NODE_INLET, NODE_OUTLET, NODE_STRUCT = 1 << 4, 1 << 5, 1 << 6
 

@@ -6,6 +6,7 @@ compiled or run) and writes:
   mrt_matrix_kernel_cu.txt  the same literal of kernel.cu:63-84
   d3q19_tables.txt          Direction[XDIM|YDIM|ZDIM][q] (main.cpp:87-105),
                             OppIndex (main.cpp:107-113)
+  common_h_constants.txt    the case numbers and node flags of common.h:26-37
 Usage: python tests/golden/make_fixtures.py [/root/reference]
 """
 import os
@@ -51,6 +52,17 @@ def main():
         for a in (0, 1, 2):
             f.write(" ".join(f"{d[a][q]:g}" for q in range(19)) + "\n")
         f.write(" ".join(str(opp[q]) for q in range(19)) + "\n")
+    common = read(ref, "common.h")
+    consts = {}
+    for name in ("CAVITY", "POISEUILLE", "NODE_FLUID", "NODE_FIXEDWALL", "NODE_MOVINGWALL", "NODE_INLET",
+                 "NODE_OUTLET", "NODE_STRUCT"):
+        m = re.search(r"#define\s+" + name + r"\s+(?:\(1 << (\d+)\)|(\d+))", common)
+        assert m, name
+        consts[name] = 1 << int(m.group(1)) if m.group(1) else int(m.group(2))
+    with open(os.path.join(HERE, "common_h_constants.txt"), "w") as f:
+        f.write("# name value, from the #defines of common.h\n")
+        for k, v in consts.items():
+            f.write(f"{k} {v}\n")
     print("fixtures written to", HERE)
 
 

@@ -121,3 +121,30 @@ int main() {
         pytest.skip("host compiler cannot build against hip_runtime.h: " + r.stderr[-300:])
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout
+
+
+def header_defines():
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for name, val in re.findall(r"#define\s+(NFLOW_\w+)\s+(\(1 << \d+\)|\(-?\d+\)|-?\d+)", txt):
+        m = re.fullmatch(r"\(1 << (\d+)\)", val)
+        out[name] = 1 << int(m.group(1)) if m else int(val.strip("()"))
+    return out
+
+
+def test_case_numbers_and_node_flags_are_the_reference_values():
+    """A reference caller copies Inputval.casetype and the node flags through
+    unchanged: the header's, the binding's and the oracle's values are
+    common.h's (tests/golden/common_h_constants.txt, extracted from it)."""
+    ref = {}
+    for line in open(os.path.join(ROOT, "tests", "golden", "common_h_constants.txt")):
+        if line.strip() and not line.startswith("#"):
+            k, v = line.split()
+            ref[k] = int(v)
+    h = header_defines()
+    assert h["NFLOW_PIPE"] == ref["POISEUILLE"] == nflow.PIPE == oracle.PIPE
+    assert h["NFLOW_CAVITY"] == ref["CAVITY"] == nflow.CAVITY == oracle.CAVITY
+    assert h["NFLOW_CHANNEL"] == nflow.CHANNEL == oracle.CHANNEL
+    assert len({h["NFLOW_PIPE"], h["NFLOW_CAVITY"], h["NFLOW_CHANNEL"]}) == 3
+    for flag in ("FLUID", "FIXEDWALL", "MOVINGWALL", "INLET", "OUTLET", "STRUCT"):
+        assert h["NFLOW_NODE_" + flag] == ref["NODE_" + flag] == getattr(nflow, "NODE_" + flag), flag

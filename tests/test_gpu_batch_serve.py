@@ -26,7 +26,7 @@ def mixed_inputs(storage=nflow.F32):
     return [
         nflow.defaults(**cfg(nx=16, ny=12, nz=10, enable_mrt=0, tau=0.6), storage=storage),
         nflow.defaults(**cfg(nx=33, ny=17, nz=9, enable_les=1, smagorinsky=0.17, tau=0.52), storage=storage),
-        nflow.defaults(**cfg(nx=40, ny=12, nz=10, casetype=0, enable_les=1, smagorinsky=0.17, tau=0.56,
+        nflow.defaults(**cfg(nx=40, ny=12, nz=10, casetype=nflow.CHANNEL, enable_les=1, smagorinsky=0.17, tau=0.56,
                              inlet_peak=0.05, perturb=0.01, seed=99), storage=storage),
         nflow.defaults(**cfg(nx=7, ny=5, nz=3, enable_mrt=1, tau=0.8), storage=storage),
         nflow.defaults(**cfg(nx=48, ny=48, nz=48, enable_les=1, smagorinsky=0.17, tau=0.6, init_lid_velocity=1),

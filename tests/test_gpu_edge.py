@@ -31,7 +31,7 @@ def test_cavity_edge_shapes_match_oracle(shape):
 @pytest.mark.parametrize("shape", [(3, 1, 1), (3, 4, 5), (3, 17, 2), (130, 3, 3)])
 def test_channel_edge_shapes_match_oracle(shape):
     nx, ny, nz = shape
-    c = cfg(nx=nx, ny=ny, nz=nz, casetype=0, enable_les=1, smagorinsky=0.17, tau=0.7, inlet_peak=0.05,
+    c = cfg(nx=nx, ny=ny, nz=nz, casetype=nflow.CHANNEL, enable_les=1, smagorinsky=0.17, tau=0.7, inlet_peak=0.05,
             perturb=0.01, seed=3)
     s = run_gpu(c, 25)
     o = run_oracle(c, 25)

@@ -38,7 +38,7 @@ def test_checkpoint_rejects_other_extents(tmp_path):
     t = run_gpu(cfg(nx=12, ny=10, nz=10), 0)
     with pytest.raises(nflow.NflowError):
         t.load_checkpoint(str(p))
-    u = run_gpu(cfg(nx=10, ny=10, nz=10, casetype=0), 0)  # same extents, channel
+    u = run_gpu(cfg(nx=10, ny=10, nz=10, casetype=nflow.CHANNEL), 0)  # same extents, channel
     with pytest.raises(nflow.NflowError, match="casetype"):
         u.load_checkpoint(str(p))
 

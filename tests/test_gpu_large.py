@@ -21,7 +21,7 @@ def test_cavity_256_mrt_les_against_oracle():
 
 
 def test_channel_512x128x128_against_oracle():
-    c = cfg(nx=512, ny=128, nz=128, casetype=0, enable_les=1, smagorinsky=0.17, tau=0.56, inlet_peak=0.05,
+    c = cfg(nx=512, ny=128, nz=128, casetype=nflow.CHANNEL, enable_les=1, smagorinsky=0.17, tau=0.56, inlet_peak=0.05,
             perturb=0.01, seed=2024)
     fp32_parity(c, 40)
 

@@ -46,7 +46,7 @@ def test_cavity_32_srt_1000_steps():
 
 
 def test_channel_matches_oracle():
-    c = cfg(nx=40, ny=12, nz=10, casetype=0, enable_mrt=1, enable_les=1, smagorinsky=0.17, tau=0.56,
+    c = cfg(nx=40, ny=12, nz=10, casetype=nflow.CHANNEL, enable_mrt=1, enable_les=1, smagorinsky=0.17, tau=0.56,
             inlet_peak=0.05, perturb=0.01, seed=1234)
     fp32_parity(c, 400)
 

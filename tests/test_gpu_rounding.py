@@ -24,7 +24,7 @@ def solve(c, storage, sr, steps):
 @pytest.mark.parametrize("storage", [nflow.F16, nflow.BF16])
 def test_stochastic_rounding_is_deterministic_and_equal_in_a_batch(storage):
     cs = [cfg(nx=33, ny=17, nz=9, enable_les=1, smagorinsky=0.17, tau=0.52, init_lid_velocity=1),
-          cfg(nx=40, ny=12, nz=10, casetype=0, enable_les=1, smagorinsky=0.17, tau=0.56, inlet_peak=0.05,
+          cfg(nx=40, ny=12, nz=10, casetype=nflow.CHANNEL, enable_les=1, smagorinsky=0.17, tau=0.56, inlet_peak=0.05,
               perturb=0.01, seed=7),
           cfg(nx=16, ny=12, nz=10, enable_mrt=0, tau=0.6, init_lid_velocity=1)]
     srs = [1, 1, 0]
