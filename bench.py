@@ -86,13 +86,19 @@ def valu_per_node_from_profiles(name):
 
 
 def cpu_model():
+    """The host CPU's model name with its family / model / stepping numbers
+    (virtual machines often report a bare "AMD EPYC")."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                info.setdefault(k.strip(), v.strip())
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    ids = [f"{k} {info[k]}" for k in ("cpu family", "model", "stepping") if k in info]
+    return f"{name} ({', '.join(ids)})" if ids else name
 
 
 def cpu_baseline(w, budget_s=15.0, single_budget_s=8.0):
@@ -133,12 +139,18 @@ def cpu_baseline(w, budget_s=15.0, single_budget_s=8.0):
 
 
 def solo_run(nflow, inp, steps):
-    """One request solved alone: create, initialise, run, diagnostics, destroy."""
+    """One request solved alone: create, initialise, run, diagnostics, destroy.
+    Returns the seconds taken and the mass (NaN for a lattice that diverged)."""
     t = time.perf_counter()
     s = nflow.Solver(inp)
     s.initialize()
     s.run(steps)
-    m, _, _ = s.diagnostics()
+    try:
+        m, _, _ = s.diagnostics()
+    except nflow.NflowError as e:
+        if e.code != nflow.EDIVERGED:
+            raise
+        m = float("nan")
     s.close()
     return time.perf_counter() - t, m
 
@@ -163,7 +175,7 @@ def bench_batch(args):
     wall = time.perf_counter() - t0
     dev_ms, _ = b.timing()
     nodes = b.nodes()
-    diverged = [i for i in range(len(inputs)) if not (abs(b.diagnostics(i)[0]) < float("inf"))]
+    diverged = b.diverged()  # the step kernel's instability flags
     b.close()
     # the same lattices one at a time with the single-lattice solver
     solo = {}
@@ -194,10 +206,14 @@ def bench_batch(args):
     }))
 
 
-def bench_serve(args):
-    """Poisson request stream served by the continuous-batching loop (FP16
-    storage), against the same requests solved one at a time with the plain
-    FP32 solver."""
+def serve_stream(seed, requests, step_scale, channel_fraction=0.5, tick=100, stochastic_rounding=True,
+                 tau_floor=0.0, oracle_check=True, progress=True):
+    """A seeded Poisson request stream served by the continuous-batching loop
+    (FP16 storage), against the same requests solved one at a time with the
+    plain FP32 solver.  Returns a dict: aggregate MLUPS, p50/p99 latency, the
+    served/serial ratio and the velocity errors checked (the four smallest and
+    the four lowest-tau 16-bit requests against their FP32 solo runs; the
+    smallest cavity also against the FP64 oracle)."""
     import threading
 
     import numpy as np
@@ -205,21 +221,22 @@ def bench_serve(args):
     import nflow
     import workloads
 
-    reqs = workloads.request_stream(seed=args.seed, count=args.requests, mean_gap_s=0.05,
-                                    channel_fraction=args.channel_fraction, step_scale=args.step_scale,
-                                    storage=nflow.F16)
+    reqs = workloads.request_stream(seed=seed, count=requests, mean_gap_s=0.05, channel_fraction=channel_fraction,
+                                    step_scale=step_scale, storage=nflow.F16)
     for r in reqs:
-        r["input"].stochastic_rounding = 1 if args.stochastic_rounding else 0
+        r["input"].stochastic_rounding = 1 if stochastic_rounding else 0
     _, _, hbm = nflow.device_info()
     budget = min(200 << 30, int(hbm * 0.75))
-    srv = nflow.Server(budget, storage=nflow.F16, max_resident=64, steps_per_tick=args.tick)
-    srv.set_tau_floor(args.tau_floor)
-    # accuracy checks against FP32 solo runs: the four smallest requests and the
-    # four lowest-tau ones that stay in 16-bit storage
+    srv = nflow.Server(budget, storage=nflow.F16, max_resident=64, steps_per_tick=tick)
+    srv.set_tau_floor(tau_floor)
     check = sorted(reqs, key=lambda r: r["nodes"])[:4]
     check += [r for r in sorted(reqs, key=lambda r: r["input"].tau)
-              if r["input"].tau >= args.tau_floor and r not in check][:4]
+              if r["input"].tau >= tau_floor and r not in check][:4]
     check_tags = {r["tag"] for r in check}
+    cavities = [r for r in reqs if "cavity" in r["kind"]]
+    smallest_cavity = min(cavities, key=lambda r: r["nodes"] * r["steps"]) if cavities else None
+    if smallest_cavity is not None:
+        check_tags.add(smallest_cavity["tag"])
     t_start = time.perf_counter()
 
     def submitter():
@@ -235,30 +252,34 @@ def bench_serve(args):
     while len(results) < len(reqs):
         for r in srv.poll(64, wait_ms=200):
             results[r.tag] = dict(status=r.status, submit=r.submit_s, admit=r.admit_s, done=r.done_s)
-            if len(results) % 8 == 0:
+            if progress and len(results) % 8 == 0:
                 print(f"served {len(results)}/{len(reqs)}", file=sys.stderr, flush=True)
     th.join()
     wall = max(v["done"] for v in results.values()) - min(v["submit"] for v in results.values())
     st = srv.stats()
-    fields = {t: srv.fetch_macro(t, (r["input"].ny, r["input"].nz, r["input"].nx))
-              for t, r in ((r["tag"], r) for r in check)}
+    fields = {}
+    for r in reqs:
+        if r["tag"] in check_tags and results[r["tag"]]["status"] == 0:
+            fields[r["tag"]] = srv.fetch_macro(r["tag"], (r["input"].ny, r["input"].nz, r["input"].nx))
     srv.close()
     lat = np.array([v["done"] - v["submit"] for v in results.values()])
     updates = sum(r["nodes"] * r["steps"] for r in reqs)
     ok = sum(1 for v in results.values() if v["status"] == 0)
+    diverged = sum(1 for v in results.values() if v["status"] == nflow.EDIVERGED)
     # the same requests, one at a time, plain FP32 solver
     serial = 0.0
     errs = []
     for k, r in enumerate(reqs):
-        if k % 8 == 0:
+        if progress and k % 8 == 0:
             print(f"serial {k}/{len(reqs)}", file=sys.stderr, flush=True)
         inp = r["input"]
         plain = nflow.defaults(**{k: getattr(inp, k) for k, _ in inp._fields_ if k != "ref_vel"},
                                ref_vel=tuple(inp.ref_vel))
         plain.storage = nflow.F32
+        plain.stochastic_rounding = 0
         dt, _ = solo_run(nflow, plain, r["steps"])
         serial += dt
-        if r["tag"] in check_tags:
+        if r["tag"] in fields:
             s = nflow.Solver(plain)
             s.initialize()
             s.run(r["steps"])
@@ -268,32 +289,57 @@ def bench_serve(args):
             u32 = np.stack([ux, uy, uz]).astype(np.float64)
             u16 = np.stack(g[1:]).astype(np.float64)
             errs.append({"request": r["kind"], "tau": round(inp.tau, 5), "steps": r["steps"],
-                         "fp32_storage": inp.tau < args.tau_floor,
+                         "fp32_storage": inp.tau < tau_floor, "reference": "FP32 solo run",
                          "rel_l2_u": float(np.linalg.norm(u16 - u32) / np.linalg.norm(u32))})
+    if oracle_check and smallest_cavity is not None and smallest_cavity["tag"] in fields:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        inp = smallest_cavity["input"]
+        L = oracle.Lattice(oracle.default_params(
+            nx=inp.nx, ny=inp.ny, nz=inp.nz, casetype=inp.casetype, tau=inp.tau, enable_mrt=inp.enable_mrt,
+            enable_les=inp.enable_les, smagorinsky=inp.smagorinsky, ref_vel=tuple(inp.ref_vel),
+            init_lid_velocity=inp.init_lid_velocity))
+        L.initialize()
+        L.run(smallest_cavity["steps"])
+        _, xo, yo, zo = L.read_macro()
+        uo = np.stack([xo, yo, zo])
+        u16 = np.stack(fields[smallest_cavity["tag"]][1:]).astype(np.float64)
+        errs.append({"request": smallest_cavity["kind"], "tau": round(inp.tau, 5), "steps": smallest_cavity["steps"],
+                     "fp32_storage": False, "reference": "FP64 oracle",
+                     "rel_l2_u": float(np.linalg.norm(u16 - uo) / np.linalg.norm(uo))})
     mlups = updates / wall / 1e6
     serial_mlups = updates / serial / 1e6
-    print(json.dumps({
-        "metric": "served-batch aggregate MLUPS and p50/p99 latency", "value": round(mlups, 1), "unit": "MLUPS",
-        "n_gpus": 1, "higher_is_better": True, "dtype": "f16-storage/f32-compute",
-        "data": f"synthetic Poisson stream, seed {args.seed}",
-        "config": {"workload": f"Poisson arrivals mean 50 ms; {1 - args.channel_fraction:g} cavity edge U[48,256] "
-                               "Re logU[100,5000] MRT-LES, "
-                               f"{args.channel_fraction:g} channel length U[256,1024] x 128 x 128 MRT-LES; "
-                               f"steps U[500,20000] x {args.step_scale}", "requests": len(reqs), "budget_bytes": budget,
-                   "steps_per_tick": args.tick},
+    return {
+        "value": round(mlups, 1), "unit": "MLUPS", "dtype": "f16-storage/f32-compute" +
+        (", stochastic rounding" if stochastic_rounding else ""),
+        "data": f"synthetic Poisson stream, seed {seed}",
+        "workload": f"{len(reqs)} requests, Poisson arrivals mean 50 ms; {1 - channel_fraction:g} cavity edge "
+                    f"U[48,256] Re logU[100,5000] MRT-LES, {channel_fraction:g} channel length U[256,1024] x 128 "
+                    f"x 128 MRT-LES; steps U[500,20000] x {step_scale}; {budget / 2**30:.0f} GiB budget",
+        "steps_per_tick": tick,
         "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
         "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
-        "completed_ok": ok, "wall_s": round(wall, 3), "node_updates": updates,
+        "completed_ok": ok, "diverged": diverged, "wall_s": round(wall, 3), "node_updates": updates,
         "serial_plain_f32_mlups": round(serial_mlups, 1), "speedup_vs_serial_plain": round(mlups / serial_mlups, 3),
         "peak_resident": st["peak_resident"], "peak_bytes": st["peak_bytes"],
         "server_busy_s": round(st["busy_s"], 3), "server_step_s": round(st["step_s"], 3), "ticks": st["ticks"],
         "step_mlups": round(st["node_updates"] / max(st["step_s"], 1e-9) / 1e6, 1),
-        "tau_floor": args.tau_floor, "stochastic_rounding": bool(args.stochastic_rounding),
-        "fp32_requests": sum(1 for r in reqs if r["input"].tau < args.tau_floor),
-        "rel_l2_u_vs_fp32_solo": errs,
-        "max_rel_l2_u_checked": max(e["rel_l2_u"] for e in errs),
-        "latency_includes": "queueing: offered load exceeds capacity",
-    }))
+        "tau_floor": tau_floor, "fp32_requests": sum(1 for r in reqs if r["input"].tau < tau_floor),
+        "rel_l2_u_checked": errs,
+        "max_rel_l2_u_checked": max(e["rel_l2_u"] for e in errs) if errs else None,
+        "latency_includes": "queueing: requests arrive faster than they are served",
+    }
+
+
+def bench_serve(args):
+    """Poisson request stream served by the continuous-batching loop (FP16
+    storage), against the same requests solved one at a time with the plain
+    FP32 solver."""
+    r = serve_stream(args.seed, args.requests, args.step_scale, args.channel_fraction, args.tick,
+                     args.stochastic_rounding, args.tau_floor)
+    print(json.dumps({"metric": "served-batch aggregate MLUPS and p50/p99 latency", "n_gpus": 1,
+                      "higher_is_better": True, **r}))
 
 
 def bench_precision(args):
@@ -407,8 +453,10 @@ def main():
     # the single-lattice solver is a batch of one; use the batch handle so the
     # launch shape can be tuned
     b = nflow.Batch([inp])
+    tuned_ms = None
     if not args.no_autotune:
         tuning = b.autotune(steps_per_trial=10)
+        tuned_ms = tuning["ms_per_step"]
         b.set_tuning(tuning["block"], tuning["xcd_order"], tuning["persistent"], args.graph_steps)
     else:
         b.set_tuning(args.block, 0, 0, args.graph_steps)
@@ -458,7 +506,9 @@ def main():
         "dtype": DTYPE[w["storage"]] + (", stochastic rounding" if w.get("sr") else ""),
         "data": "synthetic (lattice initialised at rest, rho = 1)",
         "config": {"workload": w["desc"], "nx": w["nx"], "ny": w["ny"], "nz": w["nz"], "nodes": n,
-                   "tuning": b.get_tuning(), "device": name, "cus": cus},
+                   "tuning": dict(b.get_tuning(), ms_per_step=tuned_ms, autotuned=not args.no_autotune,
+                                  graph_replay=args.graph_steps > 0 and args.steps // 5 >= args.graph_steps),
+                   "device": name, "cus": cus},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(args.workload),
                      "bytes_per_update": bpu, "kernel_ms": round(dev_ms, 5),

@@ -60,6 +60,7 @@ extern "C" {
 #define NFLOW_ESTATE (-4)  /* call out of order (e.g. run before initialize) */
 #define NFLOW_EBUDGET (-5) /* request larger than the server's HBM budget */
 #define NFLOW_ENOTFOUND (-6)
+#define NFLOW_EDIVERGED (-7) /* the lattice went unstable: some node's density was not a positive finite number */
 
 /* Parameter block.  Superset of Inputval (struct.h:78-100): the fields the
  * step uses, plus storage precision and the channel inlet description. */
@@ -84,11 +85,14 @@ typedef struct nflow_input {
   /* pipe: nodes with (y - cy)^2 + (z - cz)^2 > r^2 are solid (kernel.cu:1090
    * uses cy = cz = 25, r = 20); the inlet velocity is ref_vel (kernel.cu:601) */
   double pipe_radius, pipe_cy, pipe_cz;
-  /* FP16/BF16 storage: 1 rounds each stored f - w_q stochastically (a hash of
-   * the node's coordinates and the value's own bits picks the direction, with
-   * probability by distance) instead of to nearest, so that rounding errors
-   * do not correlate from step to step and accumulate over long runs;
-   * deterministic, ignored for FP32.  0: round to nearest (the default). */
+  /* FP16/BF16 storage: 1 rounds each stored f - w_q stochastically instead of
+   * to nearest: it goes to the grid point above with probability equal to its
+   * distance from the one below, in units of the grid spacing (exact for
+   * normal values; gfx950's v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32).  The random
+   * bits come from a hash of the node id and the collision count, so rounding
+   * errors do not correlate from step to step and average out over long runs
+   * instead of accumulating; runs stay deterministic and a lattice rounds the
+   * same way alone and in a batch.  Ignored for FP32.  0: round to nearest. */
   int stochastic_rounding;
 } nflow_input;
 
@@ -144,7 +148,12 @@ int nflow_read_node_type(nflow_solver* s, int* node_type, int* direction_state);
  * nflow_initialize (the pipe case builds its own mask). */
 int nflow_set_solid_mask(nflow_solver* s, const unsigned char* mask);
 /* FP64 reductions on the device: total mass, momentum and the residual
- * sum |PrevDen - CurrDen| of kernel.cu:241-245. */
+ * sum |PrevDen - CurrDen| of kernel.cu:241-245.  Returns NFLOW_EDIVERGED
+ * (outputs still filled) when the lattice has gone unstable: some node's
+ * density is not a positive finite number (checked over every node by this
+ * call), an earlier check since initialisation found so, or a sum is not
+ * finite.  The reference only prints the residual (kernel.cu:232-248) and
+ * runs on. */
 int nflow_diagnostics(nflow_solver* s, double* mass, double mom[3], double* drho_l1);
 /* Device time of the last nflow_run, per step (HIP events on the solver's stream). */
 int nflow_timing(nflow_solver* s, double* ms_per_step, int* steps_timed);
@@ -152,9 +161,13 @@ void nflow_destroy(nflow_solver* s);
 const char* nflow_last_error(void);
 
 /* ---- restart files (Inputval.irestart/irst, struct.h:84; kernel.cu:706) ----
- * A checkpoint is the exported arF plus the step count: loading it into a
- * solver of the same extents and case type continues the run exactly where it
- * was saved (bitwise for FP32 storage); any other solver is refused. */
+ * A checkpoint is the exported arF, the step count, the storage format and a
+ * hash of the parameters the step uses (case, collision model, tau, LES,
+ * lid/inlet description, pipe geometry, rounding mode): loading it into a
+ * solver with the same extents, storage and parameters continues the run
+ * exactly where it was saved (bitwise for FP32 storage and for stochastic
+ * rounding's step-seeded stores); any other solver is refused with
+ * NFLOW_EINVAL. */
 int nflow_save_checkpoint(nflow_solver* s, const char* path);
 int nflow_load_checkpoint(nflow_solver* s, const char* path);
 int nflow_checkpoint_info(const char* path, int* nx, int* ny, int* nz, long long* steps);
@@ -191,7 +204,14 @@ int nflow_batch_run(nflow_batch* b, int nsteps);
 int nflow_batch_synchronize(nflow_batch* b);
 int nflow_batch_read_arF(nflow_batch* b, int idx, float* dst);
 int nflow_batch_read_macro(nflow_batch* b, int idx, float* rho, float* ux, float* uy, float* uz);
+/* As nflow_diagnostics, for lattice idx (NFLOW_EDIVERGED likewise). */
 int nflow_batch_diagnostics(nflow_batch* b, int idx, double* mass, double mom[3], double* drho_l1);
+/* flags[i] = 1 for each lattice that has gone unstable, else 0: some node's
+ * rest population is not finite (checked now over every node of every
+ * lattice, one population per node read: a NaN or infinity in any population
+ * reaches it through the next collision) or an earlier check since the
+ * lattice's initialisation found it unstable (see nflow_diagnostics). */
+int nflow_batch_diverged(nflow_batch* b, int* flags);
 int nflow_batch_timing(nflow_batch* b, double* ms_per_step, int* steps_timed);
 long long nflow_batch_nodes(nflow_batch* b);
 size_t nflow_batch_device_bytes(nflow_batch* b);
@@ -212,7 +232,8 @@ typedef struct nflow_request {
 
 typedef struct nflow_result {
   unsigned long long tag;
-  int status;               /* 0 ok, negative NFLOW_E* */
+  int status;               /* 0 ok, negative NFLOW_E*; NFLOW_EDIVERGED: the request went
+                               unstable; the check after every tick stops it there */
   int nx, ny, nz;
   long long steps;
   double submit_s, admit_s, done_s; /* seconds since the server started */
@@ -255,6 +276,13 @@ int nflow_copy_bandwidth(size_t bytes, int streams, int reps, double* gbps, doub
  * launches.  *g_wave_fma_per_s = wave-level v_fma_f32 instructions per
  * second in units of 1e9 (each one 64 lane FMAs). */
 int nflow_valu_rate(int reps, double* g_wave_fma_per_s, double* ms_per_launch);
+/* The step kernel's 16-bit store conversion applied to n host floats
+ * (storage NFLOW_STORAGE_F16 or _BF16): to nearest even, or, with stochastic
+ * = 1, stochastically with seed[i] for src[i] (the rounding described at
+ * nflow_input.stochastic_rounding).  dst receives the 16-bit patterns.  For
+ * checking the rounding; n <= 2^30. */
+int nflow_round16(const float* src, const unsigned* seeds, size_t n, int storage, int stochastic,
+                  unsigned short* dst);
 int nflow_device_info(char* name, int name_len, int* cu_count, size_t* hbm_bytes);
 
 #ifdef __cplusplus

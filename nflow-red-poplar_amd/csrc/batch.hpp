@@ -77,6 +77,9 @@ class Batch {
   double* d_out5 = nullptr;
   int* d_types = nullptr;
   size_t d_types_cap = 0;
+  uint32_t* d_flags = nullptr;  // divergence flags of every lattice, gathered
+  size_t d_flags_cap = 0;
+  std::vector<uint32_t> h_flags;
 
   Batch() = default;
   Batch(const Batch&) = delete;
@@ -87,14 +90,19 @@ class Batch {
   int add_lattice(const nflow_input& in, const std::vector<Tile>& tiles, void* arena_mem);
   void remove_lattice(int idx);  // frees owned memory; the caller syncs first
   int upload_tables();
-  int init_lattice(int idx, const float* d_src_aos);
+  // step0: the step count the state stands at (0 for a new lattice)
+  int init_lattice(int idx, const float* d_src_aos, long long step0 = 0);
   int init_all();
   int launch_steps(int nsteps);  // enqueue only
   int run(int nsteps);           // enqueue with timing events
   int synchronize();
   int timing(double* ms_per_step, int* steps);
-  int export_one(int idx, float* aos, float* rho, float* ux, float* uy, float* uz, double* diag5);
-  int import_one(int idx, const float* aos_host);
+  // diverged (nullable): the lattice's divergence flag
+  int export_one(int idx, float* aos, float* rho, float* ux, float* uy, float* uz, double* diag5,
+                 int* diverged = nullptr);
+  // flags of all lattices in order (tables must be current)
+  int diverged_flags(std::vector<int>& out);
+  int import_one(int idx, const float* aos_host, long long step0);
   int node_types(int idx, int* type, int* dirstate);
   int autotune(int trial_steps, nflow_tuning* out);
   int set_tuning(const nflow_tuning& t);

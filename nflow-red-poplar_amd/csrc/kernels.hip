@@ -142,17 +142,31 @@ struct StF32 {
   static constexpr bool buffer = false;  // step-kernel access through buffer instructions
   __device__ static inline float dec(T v) { return v; }
   __device__ static inline T enc(float v) { return v; }
+  __device__ static inline T enc_sr(float v, uint32_t) { return v; }
 };
+
+// Stochastic rounding to 16 bits with gfx950's v_cvt_sr_f16_f32 /
+// v_cvt_sr_bf16_f32.  Measured behaviour of the instruction (every seed, for
+// values at and beside binade edges, subnormals and both signs): the
+// magnitude's bits below the 16-bit mantissa (13 for FP16, 16 for BF16) are
+// added to the top 13 / 16 bits of the seed and the sum is truncated, so a
+// value rounds away from zero with probability equal to its distance from the
+// grid point below it, in units of the grid spacing -- exactly, with the carry
+// into the next binade handled.  FP16 results in the subnormal range
+// (|v| < 2^-14) use the same 13 seed bits for up to 23 discarded bits: the
+// bias there is below 2^-13 of the 2^-24 spacing.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 struct StF16 {
   using T = _Float16;
   static constexpr bool shifted = true;
   static constexpr bool buffer = true;
-  static constexpr uint32_t mant = 10;              // stored mantissa bits
-  static constexpr uint32_t min_exp = 0x38800000u;  // float bits of 2^-14, the smallest normal: below it
-                                                    // the spacing stays 2^-24 (subnormals)
   __device__ static inline float dec(T v) { return (float)v; }
   __device__ static inline T enc(float v) { return (_Float16)v; }
+  __device__ static inline T enc_sr(float v, uint32_t seed) {
+    return __builtin_amdgcn_cvt_sr_f16_f32(f16x2{0, 0}, v, seed, false)[0];
+  }
 };
 
 // gfx950 converts float -> bf16 (round to nearest even) in one instruction
@@ -161,12 +175,11 @@ struct StBF16 {
   using T = __bf16;
   static constexpr bool shifted = true;
   static constexpr bool buffer = true;
-  static constexpr uint32_t mant = 7;
-  // floor of the exponent used for the rounding spacing: 2^-119, so that the
-  // spacing 2^-126 stays a normal float (values below it round as zero does)
-  static constexpr uint32_t min_exp = 0x04000000u;
   __device__ static inline float dec(T v) { return (float)v; }
   __device__ static inline T enc(float v) { return (__bf16)v; }
+  __device__ static inline T enc_sr(float v, uint32_t seed) {
+    return __builtin_amdgcn_cvt_sr_bf16_f32(bf16x2{0, 0}, v, seed, false)[0];
+  }
 };
 
 template <class St>
@@ -174,21 +187,23 @@ __device__ constexpr float shift_of(int q) {
   return St::shifted ? W[q] : 0.0f;
 }
 
-// Stochastic rounding to a 16-bit format: v plus a uniform dither in
-// (-1/2, 1/2) of the format's spacing at v, then round to nearest, rounds up
-// with probability equal to v's distance from the grid point below.  Eight
-// dither bits come from a 24-bit multiplicative hash of the value's bits and
-// a per-node, per-direction key, so the same inputs always give the same
-// result (batched and solo runs stay bitwise equal).
-template <class St>
-__device__ inline typename St::T enc_stochastic(float v, uint32_t key) {
-  const uint32_t b = __float_as_uint(v);
-  const uint32_t h = __umul24(b ^ key, 0x9E3779u);
-  const float d = (float)((h >> 8) & 0xFFu) + 0.5f;  // 0.5 .. 255.5
-  const uint32_t eb = max(b & 0x7F800000u, St::min_exp);
-  const float ulp = __uint_as_float(eb - (St::mant << 23));  // spacing of the 16-bit grid at v
-  return (typename St::T)fmaf(d, ulp * (1.0f / 256.0f), fmaf(-0.5f, ulp, v));
+// Seeds of the stochastic rounding.  Each collision draws one 32-bit hash per
+// node from (node id, collision count) and gives store q the seed
+// hash + q * 0x9E3779B9 (a Weyl sequence, whose top bits -- the ones the
+// conversion uses -- are uniform for every q).  The collision count makes the
+// rounding of a value that repeats exactly differ from step to step, and
+// since the seed depends on nothing else, a lattice rounds the same way alone
+// and in any batch.  mix32 is the "lowbias32" integer finaliser
+// (C. Wellons, 2018: xorshift-multiply rounds with full avalanche).
+__host__ __device__ inline uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7FEB352Du;
+  x ^= x >> 15;
+  x *= 0x846CA68Bu;
+  x ^= x >> 16;
+  return x;
 }
+constexpr uint32_t SR_WEYL = 0x9E3779B9u;
 
 // ---------------------------------------------------------------------------
 // Compile-time moment table
@@ -550,10 +565,11 @@ __device__ inline void collide_mrt(float f[Q], float rho, float ir, float jx, fl
 
 // Macroscopic -> side planes -> LES -> collision -> unit-stride store.
 // SR: the instantiation carries the stochastic-rounding store (taken for
-// lattices with LatticeDev::sr set); without it the code is not generated.
+// lattices with LatticeDev::sr set, seeded from sr_hash); without it the code
+// is not generated.
 template <class St, bool SR, class Acc>
 __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], const Acc& out, float* __restrict__ side_out,
-                                       uint32_t nr, int x, int y, int z) {
+                                       uint32_t nr, int x, int y, int z, uint32_t sr_hash) {
   float rho, jx, jy, jz;
   moments(f, rho, jx, jy, jz);
   const float ir = hw_rcp(rho);
@@ -572,11 +588,9 @@ __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], const Ac
   const uint32_t ps = (uint32_t)L.pstride;
   if constexpr (St::shifted && SR) {
     if (L.sr) {
-      const uint32_t key = ((uint32_t)x * 0x9E3779B1u) ^ ((uint32_t)y * 0x85EBCA77u) ^ ((uint32_t)z * 0xC2B2AE3Du);
 #pragma unroll
       for (int q = 0; q < Q; ++q)
-        out.put(pidx<typename St::T>(q, nr, ps),
-                enc_stochastic<St>(f[q] - shift_of<St>(q), key ^ ((uint32_t)q * 0x27D4EB2Fu)));
+        out.put(pidx<typename St::T>(q, nr, ps), St::enc_sr(f[q] - shift_of<St>(q), sr_hash + (uint32_t)q * SR_WEYL));
       return;
     }
   }
@@ -604,9 +618,10 @@ __device__ inline void coords(const LatticeDev& L, uint32_t n, int& x, int& y, i
 
 // One node of a work item: pull + boundary rules, collide, store.  `in` and
 // `out` address the arrays from node bn on (0: the whole array).
+// sr_key: mix32 of this collision's count (stochastic rounding only).
 template <class St, bool MASK, bool SR, class Acc>
 __device__ __forceinline__ void step_node(const LatticeDev& L, const WorkItem& wi, int p, uint32_t bn, const Acc& in,
-                                 const Acc& out) {
+                                 const Acc& out, uint32_t sr_key) {
   if (threadIdx.x >= wi.count) return;
   const uint32_t n = wi.start + threadIdx.x;
   if (MASK && is_solid(L, n)) return;  // solid nodes are never updated
@@ -614,7 +629,7 @@ __device__ __forceinline__ void step_node(const LatticeDev& L, const WorkItem& w
   coords(L, n, x, y, z);
   float f[Q];
   gather<St, MASK>(L, in, L.side[p], n, n - bn, x, y, z, f);
-  relax_and_store<St, SR>(L, f, out, L.side[p ^ 1], n - bn, x, y, z);
+  relax_and_store<St, SR>(L, f, out, L.side[p ^ 1], n - bn, x, y, z, SR ? mix32(n ^ sr_key) : 0u);
 }
 
 // MASK = true: at least one lattice of the launch has solid nodes (the plain
@@ -634,6 +649,16 @@ __global__ void __launch_bounds__(BS) NFLOW_STEP_ATTR k_step(const LatticeDev* _
     const WorkItem wi = items[it];
     const LatticeDev& L = lats[wi.lat];
     const int p = parity ^ L.parity0;
+    uint32_t sr_key = 0;
+    if constexpr (SR) {
+      // this launch rounds with collision count ctr[p] (k_advance, launched
+      // after it, hands ctr[p] + 1 to the next launch in the other slot).
+      // Nothing in this kernel writes the counters, so the read goes through
+      // the constant address space and becomes a scalar load: the step
+      // kernel carries no store besides its populations and side records,
+      // whose waits a store of its own would lengthen.
+      if (L.sr) sr_key = mix32(((__attribute__((address_space(4))) const uint32_t*)L.ctr)[p]);
+    }
     if constexpr (NFLOW_AOSOA && NFLOW_BUFFER && St::buffer) {
       constexpr uint32_t B = 256 / sizeof(T);
       const uint32_t reach = (uint32_t)L.nxz + (uint32_t)L.nx;
@@ -641,20 +666,35 @@ __global__ void __launch_bounds__(BS) NFLOW_STEP_ATTR k_step(const LatticeDev* _
       const uint64_t skip = (uint64_t)bn * Q;  // elements before the window
       const uint64_t bytes = ((uint64_t)L.pstride * Q - skip) * sizeof(T);
       step_node<St, MASK, SR>(L, wi, p, bn, BPops<T>(static_cast<T*>(L.buf[p]) + skip, bytes),
-                          BPops<T>(static_cast<T*>(L.buf[p ^ 1]) + skip, bytes));
+                          BPops<T>(static_cast<T*>(L.buf[p ^ 1]) + skip, bytes), sr_key);
     } else {
-      step_node<St, MASK, SR>(L, wi, p, 0u, GPops<T>{static_cast<T*>(L.buf[p])}, GPops<T>{static_cast<T*>(L.buf[p ^ 1])});
+      step_node<St, MASK, SR>(L, wi, p, 0u, GPops<T>{static_cast<T*>(L.buf[p])}, GPops<T>{static_cast<T*>(L.buf[p ^ 1])},
+                              sr_key);
     }
   }
+}
+
+// Collision counters of the stochastic rounding: after the launch that read
+// buf[p] (p = parity ^ parity0), ctr[p ^ 1] = ctr[p] + 1.
+__global__ void __launch_bounds__(256) k_advance(const LatticeDev* __restrict__ lats, int nlat, int parity) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nlat) return;
+  const LatticeDev& L = lats[i];
+  if (!L.sr) return;
+  const int p = parity ^ L.parity0;
+  L.ctr[p ^ 1] = L.ctr[p] + 1u;
 }
 
 // Initialisation (Initialize vertex, lbm_codelets.cpp:140-241) or import of
 // an arF array: the streamed state goes to buf[cur ^ 1] (what an export at
 // step 0 returns), its collision to buf[cur] (what the first step streams).
+// The state stands at step `step0`: its collision is rounded with that count
+// and the first step launch reads step0 + 1 from ctr[cur].
 template <class St>
-__global__ void __launch_bounds__(256) k_init(LatticeDev L, int cur, const float* __restrict__ src) {
+__global__ void __launch_bounds__(256) k_init(LatticeDev L, int cur, const float* __restrict__ src, uint32_t step0) {
   const uint32_t n = blockIdx.x * 256u + threadIdx.x;
   if (n >= L.n) return;
+  if (n == 0) L.ctr[cur] = step0 + 1u;
   int x, y, z;
   coords(L, n, x, y, z);
   if (is_solid(L, n)) {
@@ -702,7 +742,7 @@ __global__ void __launch_bounds__(256) k_init(LatticeDev L, int cur, const float
 #pragma unroll
   for (int q = 0; q < Q; ++q) streamed.put(pidx<typename St::T>(q, n, ps), St::enc(f[q] - shift_of<St>(q)));
   relax_and_store<St, true>(L, f, GPops<typename St::T>{static_cast<typename St::T*>(L.buf[cur])}, L.side[cur], n, x, y,
-                            z);
+                            z, mix32(n ^ mix32(step0)));
 }
 
 // ---------------------------------------------------------------------------
@@ -755,6 +795,9 @@ __global__ void __launch_bounds__(EXPORT_BS) k_export(LatticeDev L, int prev, in
     }
     float rho, jx, jy, jz;
     moments(f, rho, jx, jy, jz);
+    // instability: a density that is not a positive finite number marks the
+    // lattice (NaN fails both comparisons)
+    if (!(rho > 0.0f && rho < __builtin_inff())) L.ctr[CTR_DIVERGED] = 1u;
     const float ir = 1.0f / rho;
     if (aos) {
       const size_t b = (((size_t)(y + 1) * (L.nz + 2) + (z + 1)) * (L.nx + 2) + (x + 1)) * Q;
@@ -817,6 +860,32 @@ __global__ void __launch_bounds__(256) k_reduce(const double* __restrict__ parti
   }
 }
 
+// Instability probe over a batch's work list: the rest population of every
+// node in the newest state (the buffer the next step reads).  A NaN or
+// infinity in any population of a node reaches its rest population through
+// the next collision, so a lattice that blows up is caught within a step, for
+// one 19th of a buffer's traffic.  (The step kernel itself carries no check:
+// one conditional store in it measured 19% slower at FP16 storage.)
+template <class St>
+__global__ void __launch_bounds__(512) k_probe(const LatticeDev* __restrict__ lats, const WorkItem* __restrict__ items,
+                                               int nitems, int parity) {
+  using T = typename St::T;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const WorkItem wi = items[it];
+    if (threadIdx.x >= wi.count) continue;
+    const LatticeDev& L = lats[wi.lat];
+    const uint32_t n = wi.start + threadIdx.x;
+    const GPops<T> b{static_cast<T*>(L.buf[parity ^ L.parity0])};
+    const float v = St::dec(b.get(pidx<T>(0, n, (uint32_t)L.pstride)));
+    if (!__builtin_isfinite(v)) L.ctr[CTR_DIVERGED] = 1u;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_flags(const LatticeDev* __restrict__ lats, int nlat, uint32_t* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nlat) out[i] = lats[i].ctr[CTR_DIVERGED];
+}
+
 // keSetNodeType + keInitialNodeDirection (kernel.cu:1020-1097) for read-back.
 __global__ void __launch_bounds__(256) k_node_type(LatticeDev L, int* __restrict__ type, int* __restrict__ dirstate) {
   const uint32_t n = blockIdx.x * 256u + threadIdx.x;
@@ -856,6 +925,17 @@ __global__ void __launch_bounds__(256) k_copy_runs(const float* __restrict__ src
   for (int q = 0; q < Q; ++q) v[q] = __builtin_nontemporal_load((gptr<const float>)src + (blk * Q + q) * 64 + lane);
 #pragma unroll
   for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(v[q], (gptr<float>)dst + (blk * Q + q) * 64 + lane);
+}
+
+// The 16-bit store conversion of the step kernel applied to an array:
+// to nearest (enc) or stochastically with the given seeds (enc_sr).
+template <class St>
+__global__ void __launch_bounds__(256) k_round16(const float* __restrict__ src, const uint32_t* __restrict__ seeds,
+                                                 uint16_t* __restrict__ dst, size_t n, int stochastic) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const typename St::T v = stochastic ? St::enc_sr(src[i], seeds[i]) : St::enc(src[i]);
+  dst[i] = __builtin_bit_cast(uint16_t, v);
 }
 
 // VALU issue ceiling: eight independent v_fma_f32 chains per lane, 64 FMAs
@@ -909,26 +989,31 @@ static hipError_t step_mask(bool masked, int block, int grid, hipStream_t st, co
 }
 
 hipError_t launch_step(int storage, bool masked, bool sr, int block, int grid, hipStream_t st,
-                       const LatticeDev* d_lats, const WorkItem* d_items, int nitems, int parity) {
+                       const LatticeDev* d_lats, int nlat, const WorkItem* d_items, int nitems, int parity) {
   if (grid <= 0 || nitems <= 0) return hipSuccess;
+  hipError_t e = hipErrorInvalidValue;
   switch (storage) {
-    case ST_F32: return step_mask<StF32, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
+    case ST_F32: e = step_mask<StF32, false>(masked, block, grid, st, d_lats, d_items, nitems, parity); break;
     case ST_F16:
-      return sr ? step_mask<StF16, true>(masked, block, grid, st, d_lats, d_items, nitems, parity)
-                : step_mask<StF16, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
+      e = sr ? step_mask<StF16, true>(masked, block, grid, st, d_lats, d_items, nitems, parity)
+             : step_mask<StF16, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
+      break;
     case ST_BF16:
-      return sr ? step_mask<StBF16, true>(masked, block, grid, st, d_lats, d_items, nitems, parity)
-                : step_mask<StBF16, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
+      e = sr ? step_mask<StBF16, true>(masked, block, grid, st, d_lats, d_items, nitems, parity)
+             : step_mask<StBF16, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
+      break;
   }
-  return hipErrorInvalidValue;
+  if (e != hipSuccess || !sr || storage == ST_F32) return e;
+  hipLaunchKernelGGL(k_advance, dim3((unsigned)((nlat + 255) / 256)), dim3(256), 0, st, d_lats, nlat, parity);
+  return hipGetLastError();
 }
 
-hipError_t launch_init(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src) {
+hipError_t launch_init(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src, uint32_t step0) {
   const int grid = (int)((L.n + 255u) / 256u);
   switch (storage) {
-    case ST_F32: hipLaunchKernelGGL(k_init<StF32>, dim3(grid), dim3(256), 0, st, L, cur, src); break;
-    case ST_F16: hipLaunchKernelGGL(k_init<StF16>, dim3(grid), dim3(256), 0, st, L, cur, src); break;
-    case ST_BF16: hipLaunchKernelGGL(k_init<StBF16>, dim3(grid), dim3(256), 0, st, L, cur, src); break;
+    case ST_F32: hipLaunchKernelGGL(k_init<StF32>, dim3(grid), dim3(256), 0, st, L, cur, src, step0); break;
+    case ST_F16: hipLaunchKernelGGL(k_init<StF16>, dim3(grid), dim3(256), 0, st, L, cur, src, step0); break;
+    case ST_BF16: hipLaunchKernelGGL(k_init<StBF16>, dim3(grid), dim3(256), 0, st, L, cur, src, step0); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -963,6 +1048,27 @@ hipError_t launch_reduce(hipStream_t st, const double* partials, int nblocks, do
   return hipGetLastError();
 }
 
+hipError_t launch_probe(int storage, hipStream_t st, const LatticeDev* d_lats, const WorkItem* d_items, int nitems,
+                        int parity) {
+  if (nitems <= 0) return hipSuccess;
+  const int grid = nitems < 65536 ? nitems : 65536;
+  switch (storage) {
+    case ST_F32: hipLaunchKernelGGL(k_probe<StF32>, dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity); break;
+    case ST_F16: hipLaunchKernelGGL(k_probe<StF16>, dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity); break;
+    case ST_BF16:
+      hipLaunchKernelGGL(k_probe<StBF16>, dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_flags(hipStream_t st, const LatticeDev* d_lats, int nlat, uint32_t* out) {
+  if (nlat <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_flags, dim3((unsigned)((nlat + 255) / 256)), dim3(256), 0, st, d_lats, nlat, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_node_type(hipStream_t st, const LatticeDev& L, int* type, int* dirstate) {
   const int grid = (int)((L.n + 255u) / 256u);
   hipLaunchKernelGGL(k_node_type, dim3(grid), dim3(256), 0, st, L, type, dirstate);
@@ -972,6 +1078,18 @@ hipError_t launch_node_type(hipStream_t st, const LatticeDev& L, int* type, int*
 hipError_t launch_copy_runs(hipStream_t st, const float* src, float* dst, size_t nblk) {
   const size_t blocks = (nblk * 64 + 255) / 256;
   hipLaunchKernelGGL(k_copy_runs, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, nblk);
+  return hipGetLastError();
+}
+
+hipError_t launch_round16(hipStream_t st, int storage, const float* src, const uint32_t* seeds, uint16_t* dst, size_t n,
+                          int stochastic) {
+  const size_t blocks = (n + 255) / 256;
+  if (storage == ST_F16)
+    hipLaunchKernelGGL(k_round16<StF16>, dim3((unsigned)blocks), dim3(256), 0, st, src, seeds, dst, n, stochastic);
+  else if (storage == ST_BF16)
+    hipLaunchKernelGGL(k_round16<StBF16>, dim3((unsigned)blocks), dim3(256), 0, st, src, seeds, dst, n, stochastic);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

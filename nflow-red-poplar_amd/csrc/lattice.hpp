@@ -26,6 +26,12 @@ enum : int {
 // reads its inputs with one load and an outlet node with two.
 enum : int { SIDE_RHO = 0, SIDE_UIN = 1, SIDE_U0 = 4, SIDE_U1 = 8, SIDE_REC = 12 };
 
+// Words of LatticeDev::ctr: the two collision counters of the stochastic
+// rounding, then the divergence flag: set by the export when some node's
+// density is not a positive finite number and by the probe when some node's
+// rest population is not finite; cleared only by initialisation.
+enum : int { CTR_DIVERGED = 2, CTR_WORDS = 4 };
+
 // Exact unsigned division by a runtime constant d for n < 2^31:
 // floor(n / d) = (n * magic) >> shift with shift = 31 + ceil(log2 d) and
 // magic = ceil(2^shift / d) < 2^32 (Granlund & Montgomery 1994, Thm 4.2).
@@ -57,6 +63,10 @@ struct alignas(16) LatticeDev {
   void* buf[2];
   float* side[2];
   const uint8_t* mask;  // solid nodes (non-zero), or null
+  // Collision counter of the stochastic rounding, ping-pong by parity like
+  // buf: the launch reading buf[p] rounds with ctr[p]; k_advance then writes
+  // ctr[p] + 1 to ctr[p ^ 1] (k_init writes the first value).
+  uint32_t* ctr;
   long long pstride;  // elements between population planes
   int nx, ny, nz, nxz;
   uint32_t n;         // interior nodes
@@ -79,13 +89,22 @@ struct alignas(16) WorkItem {
 // Kernel launchers (kernels.hip).
 // masked: use the instantiation that honours LatticeDev::mask (any lattice of
 // the launch has solid nodes); sr: the one that can round stochastically (any
-// lattice of the launch has LatticeDev::sr set).
+// lattice of the launch has LatticeDev::sr set), followed by k_advance over the
+// nlat lattices.
 hipError_t launch_step(int storage, bool masked, bool sr, int block, int grid, hipStream_t st,
-                       const LatticeDev* d_lats, const WorkItem* d_items, int nitems, int parity);
-hipError_t launch_init(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src_aos);
+                       const LatticeDev* d_lats, int nlat, const WorkItem* d_items, int nitems, int parity);
+// step0: the step count the initialised (or imported) state stands at; the
+// stochastic rounding of its look-ahead collision uses it.
+hipError_t launch_init(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src_aos, uint32_t step0);
 hipError_t launch_export(int storage, hipStream_t st, const LatticeDev& L, int prev, int prev_streamed, float* aos,
                          float* rho, float* ux, float* uy, float* uz, double* partials, int* nblocks_out);
 hipError_t launch_reduce(hipStream_t st, const double* partials, int nblocks, double* out5);
+// rest-population probe of every node of a work list (items of at most 512
+// nodes), newest state = the buffer the launch with `parity` would read
+hipError_t launch_probe(int storage, hipStream_t st, const LatticeDev* d_lats, const WorkItem* d_items, int nitems,
+                        int parity);
+// out[i] = divergence flag of lattice i (i < nlat)
+hipError_t launch_flags(hipStream_t st, const LatticeDev* d_lats, int nlat, uint32_t* out);
 hipError_t launch_node_type(hipStream_t st, const LatticeDev& L, int* type, int* dirstate);
 hipError_t launch_copy_streams(hipStream_t st, const float* src, float* dst, size_t per_stream, int streams);
 // nblk blocks of 19 runs of 64 floats (the FP32 population layout)
@@ -93,6 +112,10 @@ hipError_t launch_copy_runs(hipStream_t st, const float* src, float* dst, size_t
 // grid workgroups of 256 lanes, each lane 64 * iters independent FMAs (out:
 // grid * 256 floats)
 hipError_t launch_fma_chains(hipStream_t st, float* out, int grid, int iters);
+// 16-bit store conversion of n floats (storage ST_F16 or ST_BF16), to
+// nearest or stochastically with per-element seeds (n < 2^32 * 256)
+hipError_t launch_round16(hipStream_t st, int storage, const float* src, const uint32_t* seeds, uint16_t* dst, size_t n,
+                          int stochastic);
 int export_block_size();
 
 }  // namespace nflow

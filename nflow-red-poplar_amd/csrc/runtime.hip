@@ -49,7 +49,7 @@ int validate_input(const nflow_input& in, int storage) {
 }
 
 struct Layout {
-  size_t pstride, pop, side, mask, total;
+  size_t pstride, pop, side, ctr, mask, total;
 };
 
 static Layout layout_of(const nflow_input& in, int storage) {
@@ -58,8 +58,9 @@ static Layout layout_of(const nflow_input& in, int storage) {
   l.pstride = round_up(n, 128) + NFLOW_PLANE_PAD;
   l.pop = round_up(19 * l.pstride * elem_size(storage), 256);
   l.side = round_up((size_t)SIDE_REC * in.ny * in.nz * sizeof(float), 256);
+  l.ctr = 256;  // collision counters of the stochastic rounding, divergence flag
   l.mask = in.casetype == NFLOW_PIPE ? round_up(n, 256) : 0;  // one byte per node
-  l.total = 2 * l.pop + 2 * l.side + l.mask;
+  l.total = 2 * l.pop + 2 * l.side + l.ctr + l.mask;
   return l;
 }
 
@@ -83,7 +84,8 @@ void lattice_layout(HostLattice& h, const nflow_input& in, int storage, void* me
   d.buf[1] = p + l.pop;
   d.side[0] = reinterpret_cast<float*>(p + 2 * l.pop);
   d.side[1] = reinterpret_cast<float*>(p + 2 * l.pop + l.side);
-  d.mask = l.mask ? reinterpret_cast<const uint8_t*>(p + 2 * l.pop + 2 * l.side) : nullptr;
+  d.ctr = reinterpret_cast<uint32_t*>(p + 2 * l.pop + 2 * l.side);
+  d.mask = l.mask ? reinterpret_cast<const uint8_t*>(p + 2 * l.pop + 2 * l.side + l.ctr) : nullptr;
   d.pstride = (long long)l.pstride;
   d.nx = in.nx;
   d.ny = in.ny;
@@ -169,6 +171,7 @@ Batch::~Batch() {
   (void)hipFree(d_partials);
   (void)hipFree(d_out5);
   (void)hipFree(d_types);
+  (void)hipFree(d_flags);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
@@ -218,7 +221,12 @@ void Batch::remove_lattice(int idx) {
   tables_dirty = true;
 }
 
+// An executable graph may still be running on the stream (its replays are
+// only enqueued), so the stream drains before the graphs are destroyed.
 void Batch::destroy_graphs() {
+  bool any = false;
+  for (auto& g : graph) any = any || g != nullptr;
+  if (any && stream) (void)hipStreamSynchronize(stream);
   for (auto& g : graph)
     if (g) {
       (void)hipGraphExecDestroy(g);
@@ -324,11 +332,13 @@ int Batch::upload_tables() {
   return 0;
 }
 
-int Batch::init_lattice(int idx, const float* d_src) {
+int Batch::init_lattice(int idx, const float* d_src, long long step0) {
   HostLattice& h = lats[idx];
-  NFLOW_HIP(launch_init(storage, stream, h.dev, gparity, d_src));
+  // counters and divergence flag start afresh (k_init sets the counter)
+  NFLOW_HIP(hipMemsetAsync(h.dev.ctr, 0, CTR_WORDS * sizeof(uint32_t), stream));
+  NFLOW_HIP(launch_init(storage, stream, h.dev, gparity, d_src, (uint32_t)step0));
   h.cur = gparity;
-  h.steps_done = 0;
+  h.steps_done = step0;
   h.prev_streamed = true;
   h.initialized = true;
   return 0;
@@ -348,7 +358,7 @@ int Batch::build_graph(int parity) {
   int p = parity;
   hipError_t err = hipSuccess;
   for (int s = 0; s < tuning.graph_steps && err == hipSuccess; ++s) {
-    err = launch_step(storage, masked, sr_any, tuning.block, grid, stream, d_lats, d_items, nitems, p);
+    err = launch_step(storage, masked, sr_any, tuning.block, grid, stream, d_lats, (int)lats.size(), d_items, nitems, p);
     p ^= 1;
   }
   hipError_t e2 = hipStreamEndCapture(stream, &g);
@@ -383,7 +393,7 @@ int Batch::launch_steps(int nsteps) {
     launches += (long long)reps * gs;
   }
   for (int s = 0; s < left; ++s) {
-    NFLOW_HIP(launch_step(storage, masked, sr_any, tuning.block, grid, stream, d_lats, d_items, nitems, gparity));
+    NFLOW_HIP(launch_step(storage, masked, sr_any, tuning.block, grid, stream, d_lats, (int)lats.size(), d_items, nitems, gparity));
     gparity ^= 1;
     ++launches;
   }
@@ -452,7 +462,32 @@ static size_t padded_count(const nflow_input& in) {
   return (size_t)(in.nx + 2) * (in.ny + 2) * (in.nz + 2) * 19;
 }
 
-int Batch::export_one(int idx, float* aos, float* rho, float* ux, float* uy, float* uz, double* diag5) {
+int Batch::diverged_flags(std::vector<int>& out) {
+  out.assign(lats.size(), 0);
+  if (lats.empty()) return 0;
+  int rc = upload_tables();
+  if (rc) return rc;
+  if (lats.size() > d_flags_cap) {
+    (void)hipFree(d_flags);
+    d_flags = nullptr;
+    if (hipMalloc(&d_flags, lats.size() * sizeof(uint32_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      d_flags_cap = 0;
+      return fail(NFLOW_ENOMEM, "divergence flags");
+    }
+    d_flags_cap = lats.size();
+  }
+  h_flags.resize(lats.size());
+  NFLOW_HIP(launch_probe(storage, stream, d_lats, d_items, nitems, gparity));
+  NFLOW_HIP(launch_flags(stream, d_lats, (int)lats.size(), d_flags));
+  NFLOW_HIP(hipMemcpyAsync(h_flags.data(), d_flags, lats.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  NFLOW_HIP(hipStreamSynchronize(stream));
+  for (size_t i = 0; i < lats.size(); ++i) out[i] = h_flags[i] ? 1 : 0;
+  return 0;
+}
+
+int Batch::export_one(int idx, float* aos, float* rho, float* ux, float* uy, float* uz, double* diag5,
+                      int* diverged) {
   if (idx < 0 || idx >= (int)lats.size()) return fail(NFLOW_EINVAL, "lattice index out of range");
   HostLattice& h = lats[idx];
   if (!h.initialized) return fail(NFLOW_ESTATE, "read before initialize");
@@ -476,17 +511,21 @@ int Batch::export_one(int idx, float* aos, float* rho, float* ux, float* uy, flo
     NFLOW_HIP(hipMemcpyAsync(uz, m + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, stream));
   }
   if (diag5) NFLOW_HIP(hipMemcpyAsync(diag5, d_out5, 5 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  uint32_t flag = 0;
+  if (diverged)
+    NFLOW_HIP(hipMemcpyAsync(&flag, h.dev.ctr + CTR_DIVERGED, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   NFLOW_HIP(hipStreamSynchronize(stream));
+  if (diverged) *diverged = flag ? 1 : 0;
   return 0;
 }
 
-int Batch::import_one(int idx, const float* aos_host) {
+int Batch::import_one(int idx, const float* aos_host, long long step0) {
   if (idx < 0 || idx >= (int)lats.size()) return fail(NFLOW_EINVAL, "lattice index out of range");
   HostLattice& h = lats[idx];
   int rc = ensure_scratch(padded_count(h.in), 0, 0, 0);
   if (rc) return rc;
   NFLOW_HIP(hipMemcpyAsync(d_aos, aos_host, padded_count(h.in) * sizeof(float), hipMemcpyHostToDevice, stream));
-  rc = init_lattice(idx, d_aos);
+  rc = init_lattice(idx, d_aos, step0);
   if (rc) return rc;
   NFLOW_HIP(hipStreamSynchronize(stream));
   return 0;
@@ -506,41 +545,50 @@ int Batch::node_types(int idx, int* type, int* dirstate) {
 }
 
 // Times every candidate launch shape on this batch's own lattices and keeps
-// the fastest.  The trial steps advance the state, so every lattice is
-// re-initialised afterwards (call before importing a state).
+// the fastest.  The clocks of an issue-bound kernel settle over the first
+// few hundred steps, which would favour whichever shapes run first, so the
+// device is warmed up with the first candidate and the candidates are timed
+// twice, in opposite orders, each keeping its better time.  The trial steps
+// advance the state, so every lattice is re-initialised afterwards (call
+// before importing a state).
 int Batch::autotune(int trial_steps, nflow_tuning* out) {
   if (trial_steps < 2) trial_steps = 2;
   const nflow_tuning base = tuning;
-  nflow_tuning best = base;
-  best.ms_per_step = 1e30;
-  const int blocks[] = {64, 128, 256, 512};
-  const int pers[] = {0, 2, 4, 8};
-  for (int bs : blocks)
+  std::vector<nflow_tuning> cand;
+  for (int bs : {64, 128, 256, 512})
     for (int xo = 0; xo < 2; ++xo)
-      for (int pe : pers) {
+      for (int pe : {0, 2, 4, 8}) {
         nflow_tuning t = base;
         t.block = bs;
         t.xcd_order = xo;
         t.persistent = pe;
         t.graph_steps = 0;
-        int rc = set_tuning(t);
-        if (rc) return rc;
-        rc = init_all();
-        if (rc) return rc;
-        rc = launch_steps(2);
-        if (rc) return rc;
-        rc = run(trial_steps);
-        if (rc) return rc;
-        double ms = 0.0;
-        rc = timing(&ms, nullptr);
-        if (rc) return rc;
-        if (ms < best.ms_per_step) {
-          best = t;
-          best.ms_per_step = ms;
-        }
+        t.ms_per_step = 1e30;
+        cand.push_back(t);
       }
+  auto trial = [&](nflow_tuning& t, int steps) -> int {
+    int rc = set_tuning(t);
+    if (!rc) rc = init_all();
+    if (!rc) rc = launch_steps(2);
+    if (!rc) rc = run(steps);
+    double ms = 0.0;
+    if (!rc) rc = timing(&ms, nullptr);
+    if (!rc) t.ms_per_step = std::min(t.ms_per_step, ms);
+    return rc;
+  };
+  nflow_tuning warm = cand.back();
+  int rc = trial(warm, 8 * trial_steps);
+  if (rc) return rc;
+  for (int pass = 0; pass < 2; ++pass)
+    for (size_t k = 0; k < cand.size(); ++k) {
+      rc = trial(cand[pass == 0 ? k : cand.size() - 1 - k], trial_steps);
+      if (rc) return rc;
+    }
+  nflow_tuning best = cand[0];
+  for (const auto& t : cand)
+    if (t.ms_per_step < best.ms_per_step) best = t;
   best.graph_steps = base.graph_steps;
-  int rc = set_tuning(best);
+  rc = set_tuning(best);
   if (rc) return rc;
   tuning.ms_per_step = best.ms_per_step;
   rc = init_all();
@@ -648,7 +696,7 @@ extern "C" int nflow_read_arF(nflow_solver* s, float* dst) {
 
 extern "C" int nflow_write_arF(nflow_solver* s, const float* src) {
   if (!s || !src) return fail(NFLOW_EINVAL, "nflow_write_arF: null argument");
-  return s->b.import_one(0, src);
+  return s->b.import_one(0, src, 0);
 }
 
 extern "C" int nflow_read_macro(nflow_solver* s, float* rho, float* ux, float* uy, float* uz) {
@@ -688,17 +736,29 @@ extern "C" int nflow_set_solid_mask(nflow_solver* s, const unsigned char* mask) 
   return 0;
 }
 
-extern "C" int nflow_diagnostics(nflow_solver* s, double* mass, double mom[3], double* drho_l1) {
-  if (!s || !mass || !mom || !drho_l1) return fail(NFLOW_EINVAL, "nflow_diagnostics: null argument");
+// Fills the FP64 sums of lattice idx; NFLOW_EDIVERGED when the lattice's
+// divergence flag is set or a sum is not finite.
+static int diagnostics_of(Batch& b, int idx, double* mass, double mom[3], double* drho_l1) {
   double d[5];
-  int rc = s->b.export_one(0, nullptr, nullptr, nullptr, nullptr, nullptr, d);
+  int diverged = 0;
+  int rc = b.export_one(idx, nullptr, nullptr, nullptr, nullptr, nullptr, d, &diverged);
   if (rc) return rc;
   *mass = d[0];
   mom[0] = d[1];
   mom[1] = d[2];
   mom[2] = d[3];
   *drho_l1 = d[4];
+  bool finite = true;
+  for (double v : d) finite = finite && std::isfinite(v);
+  if (diverged || !finite)
+    return fail(NFLOW_EDIVERGED, "lattice %d diverged: a node's density was not a positive finite number (mass %g)",
+                idx, d[0]);
   return 0;
+}
+
+extern "C" int nflow_diagnostics(nflow_solver* s, double* mass, double mom[3], double* drho_l1) {
+  if (!s || !mass || !mom || !drho_l1) return fail(NFLOW_EINVAL, "nflow_diagnostics: null argument");
+  return diagnostics_of(s->b, 0, mass, mom, drho_l1);
 }
 
 extern "C" int nflow_timing(nflow_solver* s, double* ms_per_step, int* steps_timed) {
@@ -710,22 +770,46 @@ extern "C" void nflow_destroy(nflow_solver* s) { delete s; }
 
 // ---- checkpoint files --------------------------------------------------------
 // Little-endian: 8-byte magic "NFLWCKP1", int32 header size (64), int32
-// version (1), int32 nx, ny, nz, casetype, int64 steps, int32 storage, zero
-// padding to 64 bytes, then arF as float32 [ny+2][nz+2][nx+2][19].
+// version, int32 nx, ny, nz, casetype, int64 steps, int32 storage, then
+// (version 2) uint64 params: FNV-1a of every nflow_input field the step uses,
+// zero padding to 64 bytes, then arF as float32 [ny+2][nz+2][nx+2][19].
+// Version 1 files (no params word) still load, unchecked.
 namespace {
 constexpr char CKP_MAGIC[8] = {'N', 'F', 'L', 'W', 'C', 'K', 'P', '1'};
+constexpr int32_t CKP_VERSION = 2;
+#pragma pack(push, 4)
 struct CkpHeader {
   char magic[8];
   int32_t header_bytes, version, nx, ny, nz, casetype;
   int64_t steps;
   int32_t storage;
-  char pad[20];
+  uint64_t params;
+  char pad[12];
 };
+#pragma pack(pop)
 static_assert(sizeof(CkpHeader) == 64, "checkpoint header is 64 bytes");
+
+// FNV-1a (64-bit) over the fields that change what a step computes; run
+// lengths, output cadence and the initial-state flag are left out.
+uint64_t params_hash(const nflow_input& in) {
+  uint64_t h = 0xCBF29CE484222325ull;
+  auto add = [&h](const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001B3ull;
+  };
+  const int ints[] = {in.casetype, in.enable_mrt, in.enable_les, in.stochastic_rounding};
+  const double dbl[] = {in.tau,    in.smagorinsky, in.ref_den,     in.ref_vel[0], in.ref_vel[1], in.ref_vel[2],
+                        in.w_eps,  in.w_epsj,      in.w_xx,        in.inlet_peak, in.perturb,    in.pipe_radius,
+                        in.pipe_cy, in.pipe_cz};
+  add(ints, sizeof(ints));
+  add(dbl, sizeof(dbl));
+  add(&in.seed, sizeof(in.seed));
+  return h;
+}
 
 int read_header(FILE* f, const char* path, CkpHeader& h) {
   if (std::fread(&h, sizeof(h), 1, f) != 1) return fail(NFLOW_EINVAL, "%s: short checkpoint header", path);
-  if (std::memcmp(h.magic, CKP_MAGIC, 8) != 0 || h.header_bytes != 64 || h.version != 1)
+  if (std::memcmp(h.magic, CKP_MAGIC, 8) != 0 || h.header_bytes != 64 || (h.version != 1 && h.version != CKP_VERSION))
     return fail(NFLOW_EINVAL, "%s is not an nflow checkpoint", path);
   if (h.nx < 1 || h.ny < 1 || h.nz < 1 || h.steps < 0) return fail(NFLOW_EINVAL, "%s: corrupt header", path);
   return 0;
@@ -756,13 +840,14 @@ extern "C" int nflow_save_checkpoint(nflow_solver* s, const char* path) {
   CkpHeader h{};
   std::memcpy(h.magic, CKP_MAGIC, 8);
   h.header_bytes = 64;
-  h.version = 1;
+  h.version = CKP_VERSION;
   h.nx = L.in.nx;
   h.ny = L.in.ny;
   h.nz = L.in.nz;
   h.casetype = L.in.casetype;
   h.steps = L.steps_done;
   h.storage = L.in.storage;
+  h.params = params_hash(L.in);
   FILE* f = std::fopen(path, "wb");
   if (!f) return fail(NFLOW_EINVAL, "cannot open %s for writing", path);
   const bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && std::fwrite(a.data(), sizeof(float), a.size(), f) == a.size();
@@ -782,6 +867,13 @@ extern "C" int nflow_load_checkpoint(nflow_solver* s, const char* path) {
               L.in.ny, L.in.nz);
   if (!rc && h.casetype != L.in.casetype)
     rc = fail(NFLOW_EINVAL, "%s holds casetype %d, the solver is casetype %d", path, h.casetype, L.in.casetype);
+  if (!rc && h.version >= 2 && h.storage != L.in.storage)
+    rc = fail(NFLOW_EINVAL, "%s was saved with storage %d, the solver uses storage %d", path, h.storage, L.in.storage);
+  if (!rc && h.version >= 2 && h.params != params_hash(L.in))
+    rc = fail(NFLOW_EINVAL,
+              "%s was saved with other step parameters (tau, collision model, LES, lid/inlet, pipe or rounding)", path);
+  if (!rc && (h.steps < 0 || h.steps > 0xFFFFFFFFll))
+    rc = fail(NFLOW_EINVAL, "%s: step count %lld out of range", path, (long long)h.steps);
   std::vector<float> a;
   if (!rc) {
     a.resize(padded_count(L.in));
@@ -790,10 +882,9 @@ extern "C" int nflow_load_checkpoint(nflow_solver* s, const char* path) {
   }
   std::fclose(f);
   if (rc) return rc;
-  rc = s->b.import_one(0, a.data());
-  if (rc) return rc;
-  L.steps_done = h.steps;  // the run continues from the saved step count
-  return 0;
+  // the run continues from the saved step count (which also seeds the
+  // stochastic rounding, so a restart continues it exactly)
+  return s->b.import_one(0, a.data(), h.steps);
 }
 
 // ---- batch -----------------------------------------------------------------
@@ -856,14 +947,15 @@ extern "C" int nflow_batch_read_macro(nflow_batch* b, int idx, float* rho, float
 
 extern "C" int nflow_batch_diagnostics(nflow_batch* b, int idx, double* mass, double mom[3], double* drho_l1) {
   if (!b || !mass || !mom || !drho_l1) return fail(NFLOW_EINVAL, "null argument");
-  double d[5];
-  int rc = b->b.export_one(idx, nullptr, nullptr, nullptr, nullptr, nullptr, d);
+  return diagnostics_of(b->b, idx, mass, mom, drho_l1);
+}
+
+extern "C" int nflow_batch_diverged(nflow_batch* b, int* flags) {
+  if (!b || !flags) return fail(NFLOW_EINVAL, "null argument");
+  std::vector<int> f;
+  int rc = b->b.diverged_flags(f);
   if (rc) return rc;
-  *mass = d[0];
-  mom[0] = d[1];
-  mom[1] = d[2];
-  mom[2] = d[3];
-  *drho_l1 = d[4];
+  std::copy(f.begin(), f.end(), flags);
   return 0;
 }
 
@@ -971,13 +1063,51 @@ done:
   return rc;
 }
 
+extern "C" int nflow_round16(const float* src, const unsigned* seeds, size_t n, int storage, int stochastic,
+                             unsigned short* dst) {
+  if (!src || !dst || (stochastic && !seeds)) return fail(NFLOW_EINVAL, "nflow_round16: null argument");
+  if (storage != NFLOW_STORAGE_F16 && storage != NFLOW_STORAGE_BF16)
+    return fail(NFLOW_EINVAL, "nflow_round16: storage must be NFLOW_STORAGE_F16 or NFLOW_STORAGE_BF16");
+  if (n == 0) return 0;
+  if (n > (size_t)1 << 30) return fail(NFLOW_EINVAL, "nflow_round16: at most 2^30 values per call");
+  float* d_src = nullptr;
+  uint32_t* d_seeds = nullptr;
+  uint16_t* d_dst = nullptr;
+  hipStream_t st = nullptr;
+  int rc = 0;
+  if (hipMalloc(&d_src, n * sizeof(float)) != hipSuccess || hipMalloc(&d_dst, n * sizeof(uint16_t)) != hipSuccess ||
+      (stochastic && hipMalloc(&d_seeds, n * sizeof(uint32_t)) != hipSuccess)) {
+    (void)hipGetLastError();
+    rc = fail(NFLOW_ENOMEM, "nflow_round16: device buffers");
+  } else if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+             hipMemcpyAsync(d_src, src, n * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess ||
+             (stochastic &&
+              hipMemcpyAsync(d_seeds, seeds, n * sizeof(uint32_t), hipMemcpyHostToDevice, st) != hipSuccess) ||
+             launch_round16(st, storage, d_src, d_seeds, d_dst, n, stochastic ? 1 : 0) != hipSuccess ||
+             hipMemcpyAsync(dst, d_dst, n * sizeof(uint16_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipStreamSynchronize(st) != hipSuccess) {
+    rc = fail(NFLOW_EDEVICE, "nflow_round16: %s", hipGetErrorString(hipGetLastError()));
+  }
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(d_src);
+  (void)hipFree(d_seeds);
+  (void)hipFree(d_dst);
+  return rc;
+}
+
 extern "C" int nflow_device_info(char* name, int name_len, int* cu_count, size_t* hbm_bytes) {
   int dev = 0;
   NFLOW_HIP(hipGetDevice(&dev));
   hipDeviceProp_t p;
   NFLOW_HIP(hipGetDeviceProperties(&p, dev));
   if (name && name_len > 0) {
-    std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+    // some runtimes leave the marketing name empty: name the device by its
+    // architecture, compute units and memory instead
+    if (p.name[0])
+      std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+    else
+      std::snprintf(name, (size_t)name_len, "AMD %s, %d CUs, %.0f GiB HBM", p.gcnArchName, p.multiProcessorCount,
+                    (double)p.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
   }
   if (cu_count) *cu_count = p.multiProcessorCount;
   if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;

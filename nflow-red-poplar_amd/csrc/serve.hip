@@ -10,6 +10,9 @@
 // step count (capped by steps_per_tick), so lattices leave exactly at their
 // step count; finished lattices are exported (FP64 diagnostics, optionally
 // rho/u) and their memory returns to the arena before the next admission.
+// A lattice that goes unstable (a density that is not a positive finite
+// number, flagged by the step kernel) leaves at the end of the tick that saw
+// it, with status NFLOW_EDIVERGED.
 // A lattice's arithmetic never depends on what else is resident, so a
 // request gives the same result alone and in any mix.  With a tau floor set,
 // requests whose relaxation time lies below it are kept in FP32 storage: they
@@ -17,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -131,7 +135,7 @@ class Server {
   void admit(double t);
   void reject(const Pending& p, int status, double t);
   int tick();
-  void finish(int w, size_t i);
+  void finish(int w, size_t i, bool diverged);
   size_t resident_count() const { return resident[0].size() + resident[1].size(); }
 };
 
@@ -200,7 +204,7 @@ void Server::admit(double t) {
   }
 }
 
-void Server::finish(int w, size_t i) {
+void Server::finish(int w, size_t i, bool diverged) {
   Batch& batch = batches[w];
   HostLattice& h = batch.lats[i];
   Resident& r = resident[w][i];
@@ -215,14 +219,16 @@ void Server::finish(int w, size_t i) {
   double d[5] = {0, 0, 0, 0, 0};
   std::vector<float> f;
   int rc;
-  if (r.req.want_fields) {
+  if (r.req.want_fields && !diverged) {
     const size_t n = h.dev.n;
     f.resize(4 * n);
     rc = batch.export_one((int)i, nullptr, f.data(), f.data() + n, f.data() + 2 * n, f.data() + 3 * n, d);
   } else {
     rc = batch.export_one((int)i, nullptr, nullptr, nullptr, nullptr, nullptr, d);
   }
-  res.status = rc;
+  bool finite = true;
+  for (double v : d) finite = finite && std::isfinite(v);
+  res.status = rc ? rc : (diverged || !finite) ? NFLOW_EDIVERGED : 0;
   res.mass = d[0];
   res.mom[0] = d[1];
   res.mom[1] = d[2];
@@ -230,7 +236,7 @@ void Server::finish(int w, size_t i) {
   res.drho_l1 = d[4];
   res.done_s = now();
   const long long upd = (long long)h.dev.n * h.steps_done;
-  const bool keep_fields = r.req.want_fields && rc == 0;
+  const bool keep_fields = r.req.want_fields && res.status == 0;
   void* mem = h.mem;
   const size_t bytes = h.bytes;
   batch.remove_lattice((int)i);
@@ -326,9 +332,14 @@ void Server::loop() {
       cv_done.notify_all();
       break;
     }
-    for (int w = 0; w < 2; ++w)
+    for (int w = 0; w < 2; ++w) {
+      if (resident[w].empty()) continue;
+      std::vector<int> diverged;
+      rc = batches[w].diverged_flags(diverged);
+      if (rc) diverged.assign(resident[w].size(), 0);  // the export that follows reports the device error
       for (size_t i = resident[w].size(); i-- > 0;)
-        if (batches[w].lats[i].steps_done >= resident[w][i].target) finish(w, i);
+        if (diverged[i] || batches[w].lats[i].steps_done >= resident[w][i].target) finish(w, i, diverged[i] != 0);
+    }
   }
   if (busy_from >= 0.0) {
     std::lock_guard<std::mutex> lk(mu);

@@ -16,7 +16,8 @@ PIPE, CAVITY, CHANNEL = 0, 1, 2  # POISEUILLE = 0 and CAVITY = 1 as in common.h:
 F32, F16, BF16 = 0, 1, 2
 NODE_FLUID, NODE_FIXEDWALL, NODE_MOVINGWALL = 1 << 1, 1 << 2, 1 << 3
 NODE_INLET, NODE_OUTLET, NODE_STRUCT = 1 << 4, 1 << 5, 1 << 6
-ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ESTATE", -5: "EBUDGET", -6: "ENOTFOUND"}
+ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ESTATE", -5: "EBUDGET", -6: "ENOTFOUND", -7: "EDIVERGED"}
+EDIVERGED = -7
 
 
 class NflowError(RuntimeError):
@@ -109,6 +110,7 @@ SIGNATURES = {
     "nflow_batch_read_macro": (_I, [_VP, _I, _FP, _FP, _FP, _FP]),
     "nflow_batch_diagnostics": (_I, [_VP, _I, _DP, _DP, _DP]),
     "nflow_batch_timing": (_I, [_VP, _DP, _IP]),
+    "nflow_batch_diverged": (_I, [_VP, _IP]),
     "nflow_batch_nodes": (_LL, [_VP]),
     "nflow_batch_device_bytes": (_SZ, [_VP]),
     "nflow_batch_destroy": (None, [_VP]),
@@ -122,6 +124,7 @@ SIGNATURES = {
     "nflow_server_destroy": (None, [_VP]),
     "nflow_copy_bandwidth": (_I, [_SZ, _I, _I, _DP, _DP]),
     "nflow_valu_rate": (_I, [_I, _DP, _DP]),
+    "nflow_round16": (_I, [_FP, ctypes.POINTER(ctypes.c_uint), _SZ, _I, _I, ctypes.POINTER(ctypes.c_ushort)]),
     "nflow_device_info": (_I, [ctypes.c_char_p, _I, _IP, ctypes.POINTER(ctypes.c_size_t)]),
 }
 
@@ -382,6 +385,12 @@ class Batch:
         _check(lib().nflow_batch_diagnostics(self.h, idx, ctypes.byref(m), mom, ctypes.byref(d)))
         return m.value, np.array(list(mom)), d.value
 
+    def diverged(self):
+        """Indices of the lattices that went unstable since initialisation."""
+        f = np.zeros(len(self.inputs), dtype=np.int32)
+        _check(lib().nflow_batch_diverged(self.h, _ip(f)))
+        return [int(i) for i in np.nonzero(f)[0]]
+
     def timing(self):
         ms, n = ctypes.c_double(), ctypes.c_int()
         _check(lib().nflow_batch_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
@@ -448,6 +457,25 @@ def valu_rate(reps=5):
     g, ms = ctypes.c_double(), ctypes.c_double()
     _check(lib().nflow_valu_rate(reps, ctypes.byref(g), ctypes.byref(ms)))
     return g.value, ms.value
+
+
+def round16(values, storage, seeds=None):
+    """The step kernel's 16-bit store conversion of float32 `values`: to
+    nearest even, or stochastically with uint32 `seeds` (one per value).
+    Returns the uint16 bit patterns."""
+    import numpy as np
+
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    out = np.empty(v.shape, dtype=np.uint16)
+    sp = None
+    if seeds is not None:
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32)
+        if sd.shape != v.shape:
+            raise ValueError("one seed per value")
+        sp = sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))
+    _check(lib().nflow_round16(_fp(v), sp, v.size, storage, 0 if seeds is None else 1,
+                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_ushort))))
+    return out
 
 
 def device_info():

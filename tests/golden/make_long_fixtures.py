@@ -2,7 +2,9 @@
 """Writes the full-length FP64-oracle fixtures of the GPU accuracy tests
 (tests/test_gpu_long.py): the 256^3 MRT-LES cavity at 5000 steps, the
 128^3 cavity at tau 0.51 over 5000 steps and the 512 x 128 x 128 channel at
-10000 steps, each as rho - 1 and u on a regular subsample of the box.
+10000 steps, each as rho - 1 and u on a regular subsample of the box; and
+the 128^3 case once more with the oracle's algorithm in float32 (the
+round-off yardstick of the FP32 parity bar).
 
 The oracle (oracle/liboracle.so, FP64, OpenMP) needs hours for the two large
 cases, far longer than a GPU test may take, so it runs here once and the GPU
@@ -40,6 +42,11 @@ CASES = {
         params=dict(nx=128, ny=128, nz=128, casetype=oracle.CAVITY, tau=0.51, ref_vel=(0.05, 0.0, 0.0),
                     init_lid_velocity=0, **MRT_LES),
         checkpoints={5000: 4}),
+    # the same case through the float32 build of the oracle (liboracle_f32.so)
+    "cavity128_tau0.51_f32": dict(
+        params=dict(nx=128, ny=128, nz=128, casetype=oracle.CAVITY, tau=0.51, ref_vel=(0.05, 0.0, 0.0),
+                    init_lid_velocity=0, **MRT_LES),
+        checkpoints={5000: 4}, fp32=True),
     # C3: square duct, parabolic inlet with a seeded 1% perturbation, outlet at rho = 1
     "channel512x128x128": dict(
         params=dict(nx=512, ny=128, nz=128, casetype=oracle.CHANNEL, tau=0.56, inlet_peak=0.05, perturb=0.01,
@@ -55,12 +62,14 @@ def subsample(a, k):
 def make(name):
     c = CASES[name]
     p = oracle.default_params(**c["params"])
-    L = oracle.Lattice(p)
+    fp32 = c.get("fp32", False)
+    L = oracle.Lattice(p, fp32=fp32)
     L.initialize()
     out = {"case": np.array(json.dumps({"name": name, "params": c["params"],
                                         "checkpoints": {str(k): v for k, v in c["checkpoints"].items()},
                                         "sample": "every k-th node per axis from k/2, [y][z][x]",
-                                        "oracle": "oracle/liboracle.so (FP64)"}))}
+                                        "oracle": "oracle/liboracle_f32.so (FP32)" if fp32
+                                        else "oracle/liboracle.so (FP64)"}))}
     t0 = time.time()
     done = 0
     for steps in sorted(c["checkpoints"]):

@@ -43,6 +43,24 @@ def test_checkpoint_rejects_other_extents(tmp_path):
         u.load_checkpoint(str(p))
 
 
+def test_checkpoint_rejects_other_parameters_and_storage(tmp_path):
+    c = cfg(nx=10, ny=9, nz=8, enable_les=1, smagorinsky=0.17, tau=0.6)
+    s = run_gpu(c, 4)
+    p = tmp_path / "b.ckp"
+    s.save_checkpoint(str(p))
+    for k, v in (("tau", 0.55), ("smagorinsky", 0.1), ("enable_mrt", 0), ("ref_vel", (0.04, 0.0, 0.0))):
+        t = run_gpu(dict(c, **{k: v}), 0)
+        with pytest.raises(nflow.NflowError, match="parameters"):
+            t.load_checkpoint(str(p))
+        t.close()
+    u = run_gpu(c, 0, storage=nflow.F16)
+    with pytest.raises(nflow.NflowError, match="storage"):
+        u.load_checkpoint(str(p))
+    w = run_gpu(dict(c, maxitr=999, nwrite=7), 0)  # run length and output cadence do not matter
+    w.load_checkpoint(str(p))
+    assert w.steps_done() == 4
+
+
 def test_vtu_of_device_fields(tmp_path):
     s = run_gpu(cfg(nx=9, ny=8, nz=7, init_lid_velocity=1), 5)
     r, ux, uy, uz = s.read_macro()
