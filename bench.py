@@ -32,15 +32,18 @@ WORKLOADS = {
     "cavity128": dict(nx=128, ny=128, nz=128, casetype=CAVITY, storage=0,
                       desc="3D lid-driven cavity 128^3, D3Q19 MRT + Smagorinsky LES (Cs 0.17), tau 0.6, "
                            "lid u_x 0.05, FP32 populations"),
-    "cavity256_f16": dict(nx=256, ny=256, nz=256, casetype=CAVITY, storage=1,
+    "cavity256_f16": dict(nx=256, ny=256, nz=256, casetype=CAVITY, storage=1, sr=1,
                           desc="3D lid-driven cavity 256^3, D3Q19 MRT + LES (Cs 0.17), tau 0.6, FP16 storage "
-                               "of f - w_q, FP32 arithmetic"),
-    "cavity256_f16sr": dict(nx=256, ny=256, nz=256, casetype=CAVITY, storage=1, sr=1,
+                               "of f - w_q with stochastic rounding (the default), FP32 arithmetic"),
+    "cavity256_f16rn": dict(nx=256, ny=256, nz=256, casetype=CAVITY, storage=1, sr=0,
                             desc="3D lid-driven cavity 256^3, D3Q19 MRT + LES (Cs 0.17), tau 0.6, FP16 storage "
-                                 "of f - w_q with stochastic rounding, FP32 arithmetic"),
-    "cavity256_bf16": dict(nx=256, ny=256, nz=256, casetype=CAVITY, storage=2,
+                                 "of f - w_q rounded to nearest, FP32 arithmetic"),
+    "cavity256_bf16": dict(nx=256, ny=256, nz=256, casetype=CAVITY, storage=2, sr=1,
                            desc="3D lid-driven cavity 256^3, D3Q19 MRT + LES (Cs 0.17), tau 0.6, BF16 storage "
-                                "of f - w_q, FP32 arithmetic"),
+                                "of f - w_q with stochastic rounding (the default), FP32 arithmetic"),
+    "cavity256_bf16rn": dict(nx=256, ny=256, nz=256, casetype=CAVITY, storage=2, sr=0,
+                             desc="3D lid-driven cavity 256^3, D3Q19 MRT + LES (Cs 0.17), tau 0.6, BF16 storage "
+                                  "of f - w_q rounded to nearest, FP32 arithmetic"),
     "slab256x32": dict(nx=256, ny=32, nz=256, casetype=CAVITY, storage=0,
                        desc="one GPU's share of a 256^3 cavity cut into 8 y-slabs: 256x32x256, D3Q19 MRT + LES "
                             "(Cs 0.17), tau 0.6, FP32 populations (slab faces run the wall rules)"),
@@ -60,7 +63,7 @@ def make_input(nflow, w):
     p = nflow.defaults(nx=w["nx"], ny=w["ny"], nz=w["nz"], casetype=w["casetype"], storage=w["storage"],
                        enable_mrt=1, enable_les=1, smagorinsky=0.17, tau=tau, ref_vel=(0.05, 0.0, 0.0),
                        init_lid_velocity=0, inlet_peak=0.05, perturb=0.01, seed=2024)
-    p.stochastic_rounding = w.get("sr", 0)
+    p.stochastic_rounding = w.get("sr", 1)
     return p
 
 
@@ -83,6 +86,39 @@ def valu_per_node_from_profiles(name):
     with open(path) as f:
         d = json.load(f)
     return d.get("k_step", {}).get("valu_per_node")
+
+
+def accuracy_leg(nflow, w):
+    """A fresh lattice of the benchmark workload run for 1,000 and 5,000 steps
+    and compared with the FP64 oracle's fixture of the same run (every 8th /
+    4th node per axis; tests/golden/make_long_fixtures.py wrote it): rel-L2 of
+    u and of rho - 1 at each checkpoint.  Only the 256^3 cavity has one."""
+    import numpy as np
+
+    path = os.path.join(ROOT, "tests", "golden", "long_cavity256_tau0.6.npz")
+    if not (w["nx"] == w["ny"] == w["nz"] == 256 and w["casetype"] == CAVITY):
+        return None
+    if not os.path.exists(path):
+        return {"note": f"{os.path.relpath(path, ROOT)} not generated"}
+    fx = np.load(path, allow_pickle=False)
+    cps = {int(k): v for k, v in json.loads(str(fx["case"]))["checkpoints"].items()}
+    s = nflow.Solver(make_input(nflow, w))
+    s.initialize()
+    out = {"reference": "FP64 oracle, " + os.path.relpath(path, ROOT), "steps": [], "rel_l2_u": [],
+           "rel_l2_rho_minus_1": []}
+    done = 0
+    for steps in sorted(cps):
+        s.run(steps - done)
+        done = steps
+        k = cps[steps]
+        r, ux, uy, uz = (a[k // 2::k, k // 2::k, k // 2::k].astype(np.float64) for a in s.read_macro())
+        uo = np.stack([fx[f"s{steps}_{c}"] for c in ("ux", "uy", "uz")]).astype(np.float64)
+        ro = fx[f"s{steps}_rho_minus_1"].astype(np.float64)
+        out["steps"].append(steps)
+        out["rel_l2_u"].append(float(np.linalg.norm(np.stack([ux, uy, uz]) - uo) / np.linalg.norm(uo)))
+        out["rel_l2_rho_minus_1"].append(float(np.linalg.norm(r - 1.0 - ro) / np.linalg.norm(ro)))
+    s.close()
+    return out
 
 
 def cpu_model():
@@ -337,7 +373,7 @@ def bench_serve(args):
     storage), against the same requests solved one at a time with the plain
     FP32 solver."""
     r = serve_stream(args.seed, args.requests, args.step_scale, args.channel_fraction, args.tick,
-                     args.stochastic_rounding, args.tau_floor)
+                     args.rounding == "stochastic", args.tau_floor)
     print(json.dumps({"metric": "served-batch aggregate MLUPS and p50/p99 latency", "n_gpus": 1,
                       "higher_is_better": True, **r}))
 
@@ -392,8 +428,12 @@ def bench_precision(args):
             r3, x3, y3, z3 = oracle_macro(c, steps, True)
             table.append({"case": name, "steps": steps, "storage": "oracle-f32",
                           **errors(r3, np.stack([x3, y3, z3]), ro, uo)})
-        for storage, sname in ((nflow.F32, "f32"), (nflow.F16, "f16-shifted"), (nflow.BF16, "bf16-shifted")):
-            b = nflow.Batch([nflow.defaults(storage=storage, **c)])
+        for storage, sr, sname in ((nflow.F32, 0, "f32"), (nflow.F16, 1, "f16-shifted stochastic"),
+                                   (nflow.F16, 0, "f16-shifted nearest"), (nflow.BF16, 1, "bf16-shifted stochastic"),
+                                   (nflow.BF16, 0, "bf16-shifted nearest")):
+            inp = nflow.defaults(storage=storage, **c)
+            inp.stochastic_rounding = sr
+            b = nflow.Batch([inp])
             b.set_tuning(256 if storage else 512, 0, 0, 10)
             b.initialize()
             b.run(steps)
@@ -421,6 +461,11 @@ def main():
     ap.add_argument("--cases", default="channel,cavity", help="precision mode: comma-separated channel,cavity")
     ap.add_argument("--f32-oracle", action="store_true", help="precision mode: also the FP32 build of the oracle")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the comparison with the FP64 fixture")
+    ap.add_argument("--no-serve", action="store_true", help="default workload: skip the served-stream leg")
+    ap.add_argument("--serve-requests", type=int, default=16, help="served-stream leg: requests")
+    ap.add_argument("--serve-step-scale", type=float, default=0.1, help="served-stream leg: step scale")
+    ap.add_argument("--serve-seed", type=int, default=7, help="served-stream leg: seed")
     ap.add_argument("--no-autotune", action="store_true", help="fixed launch shape instead of the autotuner's")
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--block", type=int, default=512, help="workgroup size with --no-autotune")
@@ -429,8 +474,8 @@ def main():
     ap.add_argument("--step-scale", type=float, default=0.1, help="serve mode: scale of the per-request steps")
     ap.add_argument("--tick", type=int, default=100, help="serve mode: steps per tick")
     ap.add_argument("--channel-fraction", type=float, default=0.5, help="serve mode: share of channel requests")
-    ap.add_argument("--stochastic-rounding", action="store_true",
-                    help="serve mode: round the 16-bit stores of every request stochastically")
+    ap.add_argument("--rounding", default="stochastic", choices=("stochastic", "nearest"),
+                    help="serve mode: rounding of the 16-bit stores of every request")
     ap.add_argument("--tau-floor", type=float, default=0.0,
                     help="serve mode: requests with tau below this are kept in FP32 storage")
     args = ap.parse_args()
@@ -503,7 +548,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": DTYPE[w["storage"]] + (", stochastic rounding" if w.get("sr") else ""),
+        "dtype": DTYPE[w["storage"]] + ("" if w["storage"] == 0 else
+                                        ", stochastic rounding" if w.get("sr", 1) else ", rounded to nearest"),
         "data": "synthetic (lattice initialised at rest, rho = 1)",
         "config": {"workload": w["desc"], "nx": w["nx"], "ny": w["ny"], "nz": w["nz"], "nodes": n,
                    "tuning": dict(b.get_tuning(), ms_per_step=tuned_ms, autotuned=not args.no_autotune,
@@ -520,8 +566,15 @@ def main():
         "mass_drift": abs(m - n) / n,
     }
     b.close()
+    if not args.no_accuracy:
+        acc = accuracy_leg(nflow, w)
+        if acc is not None:
+            out["accuracy"] = acc
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
+    if args.workload == "cavity256" and not args.no_serve:
+        # the metric's second half: a short served stream at real sizes
+        out["serve"] = serve_stream(args.serve_seed, args.serve_requests, args.serve_step_scale, progress=False)
     print(json.dumps(out))
 
 

@@ -92,7 +92,10 @@ typedef struct nflow_input {
    * bits come from a hash of the node id and the collision count, so rounding
    * errors do not correlate from step to step and average out over long runs
    * instead of accumulating; runs stay deterministic and a lattice rounds the
-   * same way alone and in a batch.  Ignored for FP32.  0: round to nearest. */
+   * same way alone and in a batch.  1 is the default (nflow_input_defaults):
+   * it costs ~7% of the 16-bit step rate and holds the velocity error of the
+   * long benchmark runs under 1e-3, which rounding to nearest (0) does not.
+   * Ignored for FP32. */
   int stochastic_rounding;
 } nflow_input;
 

@@ -97,9 +97,11 @@ class Batch {
   int run(int nsteps);           // enqueue with timing events
   int synchronize();
   int timing(double* ms_per_step, int* steps);
-  // diverged (nullable): the lattice's divergence flag
+  // diverged (nullable): the lattice's divergence flag.  consume: the lattice
+  // is not stepped again, so rho/u and the sums may be formed in its idle
+  // population buffer instead of allocated scratch (no aos).
   int export_one(int idx, float* aos, float* rho, float* ux, float* uy, float* uz, double* diag5,
-                 int* diverged = nullptr);
+                 int* diverged = nullptr, bool consume = false);
   // flags of all lattices in order (tables must be current)
   int diverged_flags(std::vector<int>& out);
   int import_one(int idx, const float* aos_host, long long step0);

@@ -9,10 +9,12 @@
 // type its header declares (Float32).
 // nflow_write_csv: result.csv of kernel.cu:972-997.
 #include <cstdint>
+#include <cstring>
 #include <cstdio>
 #include <string>
 #include <vector>
 
+#include "checkpoint.hpp"
 #include "errors.hpp"
 #include "nflow.h"
 
@@ -28,7 +30,9 @@ struct File {
   }
 };
 
-bool put(FILE* f, const void* p, size_t bytes) { return std::fwrite(p, 1, bytes, f) == bytes; }
+// (an empty block -- no cells in a box one node thick -- may come with a null
+// pointer, which fwrite must not see even for zero bytes)
+bool put(FILE* f, const void* p, size_t bytes) { return bytes == 0 || std::fwrite(p, 1, bytes, f) == bytes; }
 bool put_str(FILE* f, const std::string& s) { return put(f, s.data(), s.size()); }
 
 }  // namespace
@@ -132,5 +136,50 @@ extern "C" int nflow_write_csv(const char* path, int nx, int ny, int nz, const f
     ok = std::fprintf(out.f, "%d, %d, %d, %lf, %lf, %lf, %lf\n", x, y, z, rho[i], ux[i], uy[i], uz[i]) > 0;
   }
   if (!ok) return fail(NFLOW_EINVAL, "write to %s failed", path);
+  return 0;
+}
+
+// ---- checkpoint headers (format: checkpoint.hpp) -------------------------------
+namespace nflow {
+
+uint64_t params_hash(const nflow_input& in) {
+  uint64_t h = 0xCBF29CE484222325ull;
+  auto add = [&h](const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001B3ull;
+  };
+  const int ints[] = {in.casetype, in.enable_mrt, in.enable_les, in.stochastic_rounding};
+  const double dbl[] = {in.tau,    in.smagorinsky, in.ref_den,     in.ref_vel[0], in.ref_vel[1], in.ref_vel[2],
+                        in.w_eps,  in.w_epsj,      in.w_xx,        in.inlet_peak, in.perturb,    in.pipe_radius,
+                        in.pipe_cy, in.pipe_cz};
+  add(ints, sizeof(ints));
+  add(dbl, sizeof(dbl));
+  add(&in.seed, sizeof(in.seed));
+  return h;
+}
+
+int read_ckp_header(FILE* f, const char* path, CkpHeader& h) {
+  if (std::fread(&h, sizeof(h), 1, f) != 1) return fail(NFLOW_EINVAL, "%s: short checkpoint header", path);
+  if (std::memcmp(h.magic, CKP_MAGIC, 8) != 0 || h.header_bytes != 64 || (h.version != 1 && h.version != CKP_VERSION))
+    return fail(NFLOW_EINVAL, "%s is not an nflow checkpoint", path);
+  if (h.nx < 1 || h.ny < 1 || h.nz < 1 || h.steps < 0) return fail(NFLOW_EINVAL, "%s: corrupt header", path);
+  return 0;
+}
+
+}  // namespace nflow
+
+extern "C" int nflow_checkpoint_info(const char* path, int* nx, int* ny, int* nz, long long* steps) {
+  using namespace nflow;
+  if (!path) return fail(NFLOW_EINVAL, "null path");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(NFLOW_EINVAL, "cannot open %s", path);
+  CkpHeader h;
+  const int rc = read_ckp_header(f, path, h);
+  std::fclose(f);
+  if (rc) return rc;
+  if (nx) *nx = h.nx;
+  if (ny) *ny = h.ny;
+  if (nz) *nz = h.nz;
+  if (steps) *steps = h.steps;
   return 0;
 }

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "checkpoint.hpp"
 #include "d3q19.hpp"
 #include "errors.hpp"
 #include "nflow.h"
@@ -487,22 +488,39 @@ int Batch::diverged_flags(std::vector<int>& out) {
 }
 
 int Batch::export_one(int idx, float* aos, float* rho, float* ux, float* uy, float* uz, double* diag5,
-                      int* diverged) {
+                      int* diverged, bool consume) {
   if (idx < 0 || idx >= (int)lats.size()) return fail(NFLOW_EINVAL, "lattice index out of range");
   HostLattice& h = lats[idx];
   if (!h.initialized) return fail(NFLOW_ESTATE, "read before initialize");
   const size_t n = h.dev.n;
   const size_t nb = (n + export_block_size() - 1) / export_block_size();
-  int rc = ensure_scratch(aos ? padded_count(h.in) : 0, rho ? 4 * n : 0, diag5 ? 5 * nb : 0, 0);
-  if (rc) return rc;
+  float* macro = nullptr;
+  double *partials = nullptr, *out5 = nullptr;
+  // consume: the export reads buf[cur ^ 1] and the side records only, so the
+  // look-ahead state in buf[cur] (>= 38 bytes per node) can hold rho/u and the
+  // FP64 partials (16.2 bytes per node) when the lattice is not stepped again
+  const size_t b_macro = round_up(4 * n * sizeof(float), 256), b_part = round_up(5 * nb * sizeof(double), 256);
+  const size_t pop_bytes = (size_t)(static_cast<char*>(h.dev.buf[1]) - static_cast<char*>(h.dev.buf[0]));
+  if (consume && !aos && b_macro + b_part + 256 <= pop_bytes) {
+    char* base = static_cast<char*>(h.dev.buf[h.cur]);
+    macro = reinterpret_cast<float*>(base);
+    partials = reinterpret_cast<double*>(base + b_macro);
+    out5 = reinterpret_cast<double*>(base + b_macro + b_part);
+  } else {
+    int rc = ensure_scratch(aos ? padded_count(h.in) : 0, rho ? 4 * n : 0, diag5 ? 5 * nb : 0, 0);
+    if (rc) return rc;
+    macro = d_macro;
+    partials = d_partials;
+    out5 = d_out5;
+  }
   if (aos) NFLOW_HIP(hipMemsetAsync(d_aos, 0, padded_count(h.in) * sizeof(float), stream));
   const int prev = h.cur ^ 1;
   const int prev_streamed = h.prev_streamed ? 1 : 0;
   int nblocks = 0;
-  float* m = rho ? d_macro : nullptr;
+  float* m = rho ? macro : nullptr;
   NFLOW_HIP(launch_export(storage, stream, h.dev, prev, prev_streamed, aos ? d_aos : nullptr, m, m ? m + n : nullptr,
-                          m ? m + 2 * n : nullptr, m ? m + 3 * n : nullptr, diag5 ? d_partials : nullptr, &nblocks));
-  if (diag5) NFLOW_HIP(launch_reduce(stream, d_partials, nblocks, d_out5));
+                          m ? m + 2 * n : nullptr, m ? m + 3 * n : nullptr, diag5 ? partials : nullptr, &nblocks));
+  if (diag5) NFLOW_HIP(launch_reduce(stream, partials, nblocks, out5));
   if (aos) NFLOW_HIP(hipMemcpyAsync(aos, d_aos, padded_count(h.in) * sizeof(float), hipMemcpyDeviceToHost, stream));
   if (rho) {
     NFLOW_HIP(hipMemcpyAsync(rho, m, n * sizeof(float), hipMemcpyDeviceToHost, stream));
@@ -510,7 +528,7 @@ int Batch::export_one(int idx, float* aos, float* rho, float* ux, float* uy, flo
     NFLOW_HIP(hipMemcpyAsync(uy, m + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost, stream));
     NFLOW_HIP(hipMemcpyAsync(uz, m + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, stream));
   }
-  if (diag5) NFLOW_HIP(hipMemcpyAsync(diag5, d_out5, 5 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  if (diag5) NFLOW_HIP(hipMemcpyAsync(diag5, out5, 5 * sizeof(double), hipMemcpyDeviceToHost, stream));
   uint32_t flag = 0;
   if (diverged)
     NFLOW_HIP(hipMemcpyAsync(&flag, h.dev.ctr + CTR_DIVERGED, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
@@ -768,69 +786,7 @@ extern "C" int nflow_timing(nflow_solver* s, double* ms_per_step, int* steps_tim
 
 extern "C" void nflow_destroy(nflow_solver* s) { delete s; }
 
-// ---- checkpoint files --------------------------------------------------------
-// Little-endian: 8-byte magic "NFLWCKP1", int32 header size (64), int32
-// version, int32 nx, ny, nz, casetype, int64 steps, int32 storage, then
-// (version 2) uint64 params: FNV-1a of every nflow_input field the step uses,
-// zero padding to 64 bytes, then arF as float32 [ny+2][nz+2][nx+2][19].
-// Version 1 files (no params word) still load, unchecked.
-namespace {
-constexpr char CKP_MAGIC[8] = {'N', 'F', 'L', 'W', 'C', 'K', 'P', '1'};
-constexpr int32_t CKP_VERSION = 2;
-#pragma pack(push, 4)
-struct CkpHeader {
-  char magic[8];
-  int32_t header_bytes, version, nx, ny, nz, casetype;
-  int64_t steps;
-  int32_t storage;
-  uint64_t params;
-  char pad[12];
-};
-#pragma pack(pop)
-static_assert(sizeof(CkpHeader) == 64, "checkpoint header is 64 bytes");
-
-// FNV-1a (64-bit) over the fields that change what a step computes; run
-// lengths, output cadence and the initial-state flag are left out.
-uint64_t params_hash(const nflow_input& in) {
-  uint64_t h = 0xCBF29CE484222325ull;
-  auto add = [&h](const void* p, size_t n) {
-    const unsigned char* b = static_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001B3ull;
-  };
-  const int ints[] = {in.casetype, in.enable_mrt, in.enable_les, in.stochastic_rounding};
-  const double dbl[] = {in.tau,    in.smagorinsky, in.ref_den,     in.ref_vel[0], in.ref_vel[1], in.ref_vel[2],
-                        in.w_eps,  in.w_epsj,      in.w_xx,        in.inlet_peak, in.perturb,    in.pipe_radius,
-                        in.pipe_cy, in.pipe_cz};
-  add(ints, sizeof(ints));
-  add(dbl, sizeof(dbl));
-  add(&in.seed, sizeof(in.seed));
-  return h;
-}
-
-int read_header(FILE* f, const char* path, CkpHeader& h) {
-  if (std::fread(&h, sizeof(h), 1, f) != 1) return fail(NFLOW_EINVAL, "%s: short checkpoint header", path);
-  if (std::memcmp(h.magic, CKP_MAGIC, 8) != 0 || h.header_bytes != 64 || (h.version != 1 && h.version != CKP_VERSION))
-    return fail(NFLOW_EINVAL, "%s is not an nflow checkpoint", path);
-  if (h.nx < 1 || h.ny < 1 || h.nz < 1 || h.steps < 0) return fail(NFLOW_EINVAL, "%s: corrupt header", path);
-  return 0;
-}
-}  // namespace
-
-extern "C" int nflow_checkpoint_info(const char* path, int* nx, int* ny, int* nz, long long* steps) {
-  if (!path) return fail(NFLOW_EINVAL, "null path");
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return fail(NFLOW_EINVAL, "cannot open %s", path);
-  CkpHeader h;
-  const int rc = read_header(f, path, h);
-  std::fclose(f);
-  if (rc) return rc;
-  if (nx) *nx = h.nx;
-  if (ny) *ny = h.ny;
-  if (nz) *nz = h.nz;
-  if (steps) *steps = h.steps;
-  return 0;
-}
-
+// ---- checkpoint files (format and header parsing: checkpoint.hpp, io.cpp) ----
 extern "C" int nflow_save_checkpoint(nflow_solver* s, const char* path) {
   if (!s || !path) return fail(NFLOW_EINVAL, "nflow_save_checkpoint: null argument");
   const HostLattice& L = s->b.lats[0];
@@ -861,7 +817,7 @@ extern "C" int nflow_load_checkpoint(nflow_solver* s, const char* path) {
   FILE* f = std::fopen(path, "rb");
   if (!f) return fail(NFLOW_EINVAL, "cannot open %s", path);
   CkpHeader h;
-  int rc = read_header(f, path, h);
+  int rc = read_ckp_header(f, path, h);
   if (!rc && (h.nx != L.in.nx || h.ny != L.in.ny || h.nz != L.in.nz))
     rc = fail(NFLOW_EINVAL, "%s holds a %dx%dx%d lattice, the solver is %dx%dx%d", path, h.nx, h.ny, h.nz, L.in.nx,
               L.in.ny, L.in.nz);

@@ -219,12 +219,15 @@ void Server::finish(int w, size_t i, bool diverged) {
   double d[5] = {0, 0, 0, 0, 0};
   std::vector<float> f;
   int rc;
+  // the lattice leaves: the export works inside its own (idle) population
+  // buffer, so finishing allocates no device memory outside the arena
   if (r.req.want_fields && !diverged) {
     const size_t n = h.dev.n;
     f.resize(4 * n);
-    rc = batch.export_one((int)i, nullptr, f.data(), f.data() + n, f.data() + 2 * n, f.data() + 3 * n, d);
+    rc = batch.export_one((int)i, nullptr, f.data(), f.data() + n, f.data() + 2 * n, f.data() + 3 * n, d, nullptr,
+                          true);
   } else {
-    rc = batch.export_one((int)i, nullptr, nullptr, nullptr, nullptr, nullptr, d);
+    rc = batch.export_one((int)i, nullptr, nullptr, nullptr, nullptr, nullptr, d, nullptr, true);
   }
   bool finite = true;
   for (double v : d) finite = finite && std::isfinite(v);

@@ -35,6 +35,7 @@ extern "C" void nflow_input_defaults(nflow_input* in) {
   in->pipe_radius = 20.0;  // kernel.cu:1090: (y-25)^2 + (z-25)^2 > 400
   in->pipe_cy = 25.0;
   in->pipe_cz = 25.0;
+  in->stochastic_rounding = 1;  // 16-bit storage rounds stochastically; FP32 ignores it
 }
 
 // decomposition.cpp:98-161: `dec` equal blocks of n/dec nodes per axis, the

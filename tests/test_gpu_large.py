@@ -61,7 +61,9 @@ def test_largest_accepted_lattice_matches_a_short_box(storage):
     mass sums must differ by exactly the added rest planes."""
     nx, ny, nz = 512, 862, 512
     assert 19 * -(-nx * ny * nz // 128) * 128 < 2 ** 32 <= 19 * -(-nx * (ny + 1) * nz // 128) * 128
-    c = dict(nx=nx, nz=nz, enable_les=1, smagorinsky=0.17, tau=0.6)
+    # rounding to nearest: stochastic rounding seeds each node by its id, which
+    # differs between the two boxes at the same place under the lid
+    c = dict(nx=nx, nz=nz, enable_les=1, smagorinsky=0.17, tau=0.6, stochastic_rounding=0)
     small = run_gpu(cfg(ny=8, **c), 2, storage=storage)
     ms = small.diagnostics()[0]
     sm = small.read_macro()

@@ -56,7 +56,9 @@ def test_stochastic_rounding_restarts_exactly_from_a_checkpoint(tmp_path):
     a, b = t.read_arF(), full.read_arF()
     assert np.array_equal(a, b), float(np.abs(a - b).max())
     # a solver that rounds to nearest refuses the stochastic-rounding checkpoint
-    u = nflow.Solver(nflow.defaults(**c, storage=nflow.F16))
+    v = nflow.defaults(**c, storage=nflow.F16)
+    v.stochastic_rounding = 0
+    u = nflow.Solver(v)
     u.initialize()
     with pytest.raises(nflow.NflowError, match="parameters"):
         u.load_checkpoint(str(p))
