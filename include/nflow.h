@@ -158,6 +158,15 @@ int nflow_set_solid_mask(nflow_solver* s, const unsigned char* mask);
  * finite.  The reference only prints the residual (kernel.cu:232-248) and
  * runs on. */
 int nflow_diagnostics(nflow_solver* s, double* mass, double mom[3], double* drho_l1);
+/* nflow_diagnostics without allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  The same kernels and
+ * the same sums, bit for bit; the FP64 partial sums live in a static device
+ * array of the library (5 MiB per device, held until the process ends), so
+ * lattices of more than 2^25 = 33,554,432 nodes are refused with NFLOW_EINVAL.
+ * Calls on one device share that array: calls that overlap on a device race,
+ * so callers serialise them. */
+int nflow_diagnostics_noalloc(nflow_solver* s, double* mass, double mom[3], double* drho_l1);
 /* Device time of the last nflow_run, per step (HIP events on the solver's stream). */
 int nflow_timing(nflow_solver* s, double* ms_per_step, int* steps_timed);
 void nflow_destroy(nflow_solver* s);

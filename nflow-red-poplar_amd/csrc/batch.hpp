@@ -46,6 +46,7 @@ class Batch {
   int storage = NFLOW_STORAGE_F32;
   std::vector<HostLattice> lats;
   hipStream_t stream = nullptr;
+  int device = 0;  // the device current at open(), which owns the stream
   int cu_count = 256;
   nflow_tuning tuning{256, 0, 0, 10, 0.0};
 

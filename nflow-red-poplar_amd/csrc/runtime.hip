@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "batch.hpp"
@@ -181,6 +182,7 @@ Batch::~Batch() {
 int Batch::open() {
   int dev = 0;
   NFLOW_HIP(hipGetDevice(&dev));
+  device = dev;
   NFLOW_HIP(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev));
   NFLOW_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   NFLOW_HIP(hipEventCreate(&ev0));
@@ -777,6 +779,84 @@ static int diagnostics_of(Batch& b, int idx, double* mass, double mom[3], double
 extern "C" int nflow_diagnostics(nflow_solver* s, double* mass, double mom[3], double* drho_l1) {
   if (!s || !mass || !mom || !drho_l1) return fail(NFLOW_EINVAL, "nflow_diagnostics: null argument");
   return diagnostics_of(s->b, 0, mass, mom, drho_l1);
+}
+
+// ---- diagnostics without allocation -----------------------------------------
+// The FP64 block partials of the export and the five reduced sums live in a
+// file-scope device array instead of library-allocated scratch.  Its name is a
+// host-side stand-in: the array's address on a device comes from
+// hipGetSymbolAddress, called once per device (with that device current) on
+// the first call there, and only that address reaches kernels and copies.
+// Every call on a device shares its array; each call rewrites the partials of
+// the blocks it reduces and the five sums, so nothing needs clearing, but
+// calls that overlap on one device race.
+namespace {
+constexpr size_t NOALLOC_MAX_NODES = size_t(1) << 25;  // 33,554,432 nodes
+constexpr int NOALLOC_MAX_DEVICES = 64;
+}  // namespace
+static __device__ __align__(256) double scratch[(NOALLOC_MAX_NODES / 256) * 5 + 8];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int scratch_on(int device, double** out) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+    return fail(NFLOW_EINVAL, "device %d out of the %d devices the static scratch supports", device,
+                NOALLOC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    NFLOW_HIP(hipGetDevice(&cur));
+    if (cur != device) NFLOW_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(NFLOW_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+}  // namespace
+
+extern "C" int nflow_diagnostics_noalloc(nflow_solver* s, double* mass, double mom[3], double* drho_l1) {
+  if (!s || !mass || !mom || !drho_l1) return fail(NFLOW_EINVAL, "nflow_diagnostics_noalloc: null argument");
+  Batch& b = s->b;
+  const HostLattice& h = b.lats[0];
+  if (!h.initialized) return fail(NFLOW_ESTATE, "read before initialize");
+  if ((size_t)h.dev.n > NOALLOC_MAX_NODES)
+    return fail(NFLOW_EINVAL,
+                "nflow_diagnostics_noalloc: the lattice has %u nodes, more than the %zu its static scratch holds "
+                "(nflow_diagnostics has no limit)",
+                h.dev.n, NOALLOC_MAX_NODES);
+  double* ptr = nullptr;
+  int rc = scratch_on(b.device, &ptr);
+  if (rc) return rc;
+  double* const partials = ptr;
+  double* const sums = ptr + (NOALLOC_MAX_NODES / 256) * 5;
+  // the kernels, grid and order of nflow_diagnostics: the same sums, bit for
+  // bit; n <= 2^25 keeps the export's ceil(n / 256) block partials inside
+  int nblocks = 0;
+  NFLOW_HIP(launch_export(b.storage, b.stream, h.dev, h.cur ^ 1, h.prev_streamed ? 1 : 0, nullptr, nullptr, nullptr,
+                          nullptr, nullptr, partials, &nblocks));
+  NFLOW_HIP(launch_reduce(b.stream, partials, nblocks, sums));
+  double d[5];
+  uint32_t flag = 0;
+  NFLOW_HIP(hipMemcpyAsync(d, sums, sizeof(d), hipMemcpyDeviceToHost, b.stream));
+  NFLOW_HIP(hipMemcpyAsync(&flag, h.dev.ctr + CTR_DIVERGED, sizeof(flag), hipMemcpyDeviceToHost, b.stream));
+  NFLOW_HIP(hipStreamSynchronize(b.stream));
+  *mass = d[0];
+  mom[0] = d[1];
+  mom[1] = d[2];
+  mom[2] = d[3];
+  *drho_l1 = d[4];
+  bool finite = true;
+  for (double v : d) finite = finite && std::isfinite(v);
+  if (flag || !finite)
+    return fail(NFLOW_EDIVERGED, "lattice diverged: a node's density was not a positive finite number (mass %g)", d[0]);
+  return 0;
 }
 
 extern "C" int nflow_timing(nflow_solver* s, double* ms_per_step, int* steps_timed) {

@@ -91,6 +91,7 @@ SIGNATURES = {
     "nflow_read_node_type": (_I, [_VP, _IP, _IP]),
     "nflow_set_solid_mask": (_I, [_VP, ctypes.c_void_p]),
     "nflow_diagnostics": (_I, [_VP, _DP, _DP, _DP]),
+    "nflow_diagnostics_noalloc": (_I, [_VP, _DP, _DP, _DP]),
     "nflow_timing": (_I, [_VP, _DP, _IP]),
     "nflow_destroy": (None, [_VP]),
     "nflow_last_error": (ctypes.c_char_p, []),
@@ -291,6 +292,13 @@ class Solver:
         m, d = ctypes.c_double(), ctypes.c_double()
         mom = (ctypes.c_double * 3)()
         _check(lib().nflow_diagnostics(self.h, ctypes.byref(m), mom, ctypes.byref(d)))
+        return m.value, np.array(list(mom)), d.value
+
+    def diagnostics_noalloc(self):
+        """diagnostics() through nflow_diagnostics_noalloc (static device scratch)."""
+        m, d = ctypes.c_double(), ctypes.c_double()
+        mom = (ctypes.c_double * 3)()
+        _check(lib().nflow_diagnostics_noalloc(self.h, ctypes.byref(m), mom, ctypes.byref(d)))
         return m.value, np.array(list(mom)), d.value
 
     def timing(self):
