@@ -468,6 +468,8 @@ def main():
     ap.add_argument("--serve-seed", type=int, default=7, help="served-stream leg: seed")
     ap.add_argument("--no-autotune", action="store_true", help="fixed launch shape instead of the autotuner's")
     ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--temporal", type=int, default=0,
+                    help="two steps per launch, y swept in chunks of this many planes (0: one step per launch)")
     ap.add_argument("--block", type=int, default=512, help="workgroup size with --no-autotune")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--requests", type=int, default=48, help="serve mode: length of the request stream")
@@ -502,9 +504,9 @@ def main():
     if not args.no_autotune:
         tuning = b.autotune(steps_per_trial=10)
         tuned_ms = tuning["ms_per_step"]
-        b.set_tuning(tuning["block"], tuning["xcd_order"], tuning["persistent"], args.graph_steps)
+        b.set_tuning(tuning["block"], tuning["xcd_order"], tuning["persistent"], args.graph_steps, args.temporal)
     else:
-        b.set_tuning(args.block, 0, 0, args.graph_steps)
+        b.set_tuning(args.block, 0, 0, args.graph_steps, args.temporal)
     b.initialize()
     b.run(args.warmup)
     b.synchronize()

@@ -204,6 +204,12 @@ typedef struct nflow_tuning {
   int xcd_order;   /* 1: each XCD receives a contiguous range of work items */
   int persistent;  /* 0: one workgroup per work item; k > 0: k workgroups per CU, grid-stride */
   int graph_steps; /* steps per captured graph (even), 0 = no graph */
+  /* H > 0: a batch of one FP32 cavity lattice without solid nodes advances two
+   * steps per launch (x-z tiles of 256 x 8 nodes swept along y in chunks of H
+   * planes; the last launch of each run stays a single step), with results bit
+   * for bit those of one step per launch; other batches ignore it.
+   * 0 = one step per launch */
+  int temporal;
   double ms_per_step; /* filled by the autotuner */
 } nflow_tuning;
 

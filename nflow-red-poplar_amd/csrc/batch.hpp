@@ -48,7 +48,7 @@ class Batch {
   hipStream_t stream = nullptr;
   int device = 0;  // the device current at open(), which owns the stream
   int cu_count = 256;
-  nflow_tuning tuning{256, 0, 0, 10, 0.0};
+  nflow_tuning tuning{256, 0, 0, 10, 0, 0.0};
 
   // device tables
   LatticeDev* d_lats = nullptr;

@@ -38,6 +38,12 @@
 #define NFLOW_WAVES_PER_EU 0
 #endif
 
+// Which object of the library this compilation builds: 1 = the FP32 kernels
+// and the launch entry points, 2 = the FP16 / BF16 kernels (see Launchers).
+#ifndef NFLOW_STORAGE_SET
+#define NFLOW_STORAGE_SET 1
+#endif
+
 // Population layout.  NFLOW_AOSOA = 1 (default): blocks of B consecutive
 // nodes, each block holding its 19 populations as 19 runs of B elements with
 // B * sizeof(element) = 256 bytes (B = 64 at FP32, 128 at FP16/BF16).  A
@@ -273,6 +279,27 @@ __device__ inline void pull(const LatticeDev& L, const Acc& in, uint32_t n, uint
     const uint32_t idx =
         out ? pidx<typename St::T>(OPP[q], nr, ps) : pidx<typename St::T>(q, (uint32_t)((int)nr - off), ps);
     f[q] = St::dec(in.get(idx)) + shift_of<St>(q);
+  }
+}
+
+// pull() for a lattice without solid nodes, without the conversion to float:
+// the 19 stored values, so that a caller can issue the loads well ahead of
+// their use (the same elements as pull<St, false>(L, in, n, n, ...)).
+template <class St, class Acc>
+__device__ inline void pull_raw(const LatticeDev& L, const Acc& in, uint32_t n, int x, int y, int z,
+                                typename St::T r[Q]) {
+  const bool xlo = x == 0, xhi = x == L.nx - 1;
+  const bool ylo = y == 0, yhi = y == L.ny - 1;
+  const bool zlo = z == 0, zhi = z == L.nz - 1;
+  const uint32_t ps = (uint32_t)L.pstride;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const bool out = (EX[q] > 0 && xlo) || (EX[q] < 0 && xhi) || (EY[q] > 0 && ylo) || (EY[q] < 0 && yhi) ||
+                     (EZ[q] > 0 && zlo) || (EZ[q] < 0 && zhi);
+    const int off = EX[q] + EZ[q] * L.nx + EY[q] * L.nxz;
+    const uint32_t idx =
+        out ? pidx<typename St::T>(OPP[q], n, ps) : pidx<typename St::T>(q, (uint32_t)((int)n - off), ps);
+    r[q] = in.get(idx);
   }
 }
 
@@ -563,17 +590,15 @@ __device__ inline void collide_mrt(float f[Q], float rho, float ir, float jx, fl
   }
 }
 
-// Macroscopic -> side planes -> LES -> collision -> unit-stride store.
-// SR: the instantiation carries the stochastic-rounding store (taken for
-// lattices with LatticeDev::sr set, seeded from sr_hash); without it the code
-// is not generated.
-template <class St, bool SR, class Acc>
-__device__ inline void relax_and_store(const LatticeDev& L, float f[Q], const Acc& out, float* __restrict__ side_out,
-                                       uint32_t nr, int x, int y, int z, uint32_t sr_hash) {
+// Macroscopic -> side planes -> LES -> collision, in place.
+// SIDES = false: for cavity lattices only (no side records to write).
+template <bool SIDES = true>
+__device__ __forceinline__ void relax(const LatticeDev& L, float f[Q], float* __restrict__ side_out, int x, int y,
+                                      int z) {
   float rho, jx, jy, jz;
   moments(f, rho, jx, jy, jz);
   const float ir = hw_rcp(rho);
-  if (L.casetype != CASE_CAVITY) {  // channel and pipe: inlet / outlet side records
+  if (SIDES && L.casetype != CASE_CAVITY) {  // channel and pipe: inlet / outlet side records
     const gptr<float> rec = (gptr<float>)side_out + (size_t)(y * L.nz + z) * SIDE_REC;
     if (x == 1) rec[SIDE_RHO] = rho;
     if (x == L.nx - 1) *(gptr<f32x4>)(rec + SIDE_U0) = f32x4{jx * ir, jy * ir, jz * ir, 0.0f};
@@ -585,6 +610,16 @@ __device__ inline void relax_and_store(const LatticeDev& L, float f[Q], const Ac
     const float te = L.les ? tau_eddy_les(f, rho, ir, jx, jy, jz, L.tau, L.les_c) : 0.0f;
     collide_srt(f, rho, ir, jx, jy, jz, hw_rcp(L.tau + te));
   }
+}
+
+// relax, then the unit-stride store of the 19 post-collision populations.
+// SR: the instantiation carries the stochastic-rounding store (taken for
+// lattices with LatticeDev::sr set, seeded from sr_hash); without it the code
+// is not generated.
+template <class St, bool SR, bool SIDES = true, class Acc>
+__device__ inline void relax_and_store(const LatticeDev& L, float f[Q], const Acc& out, float* __restrict__ side_out,
+                                       uint32_t nr, int x, int y, int z, uint32_t sr_hash) {
+  relax<SIDES>(L, f, side_out, x, y, z);
   const uint32_t ps = (uint32_t)L.pstride;
   if constexpr (St::shifted && SR) {
     if (L.sr) {
@@ -676,6 +711,7 @@ __global__ void __launch_bounds__(BS) NFLOW_STEP_ATTR k_step(const LatticeDev* _
 
 // Collision counters of the stochastic rounding: after the launch that read
 // buf[p] (p = parity ^ parity0), ctr[p ^ 1] = ctr[p] + 1.
+#if NFLOW_STORAGE_SET != 2  // the FP32 object carries it
 __global__ void __launch_bounds__(256) k_advance(const LatticeDev* __restrict__ lats, int nlat, int parity) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nlat) return;
@@ -683,6 +719,217 @@ __global__ void __launch_bounds__(256) k_advance(const LatticeDev* __restrict__ 
   if (!L.sr) return;
   const int p = parity ^ L.parity0;
   L.ctr[p ^ 1] = L.ctr[p] + 1u;
+}
+#endif
+
+// ---------------------------------------------------------------------------
+// Two time steps per launch (a cavity lattice without solid nodes).
+//
+// A workgroup owns a TX x TZ tile of the x-z plane and a range [ya, yb) of y
+// planes, and sweeps it in y.  TX spans whole x rows of the benchmark sizes:
+// the pulls of a 128-byte line granularity make every x edge of a tile cost
+// about 28 extra lines per row, so the tile has none where nx <= TX.  Thread
+// t owns K nodes of column x0 + t % TX, stacked in z.  For each plane yi:
+//   1. the first of the two steps (pull from buf[p] with the boundary rules,
+//      collision) on the own nodes and on the halo around the tile (the rows
+//      z0 - 1 and z0 + TZ, and the columns x0 - 1 and x0 + TX where they are
+//      inside the box), spread over the threads; populations that cross to
+//      another x-z column go to LDS (from the halo rows only the 5 that cross
+//      into the tile in z); the own node's rest population, e = (0, +-1, 0)
+//      and the bounce-back of its links that leave the box go straight to its
+//      registers;
+//   2. each thread collects the crossing populations its nodes pull from
+//      plane yi: e_y = -1 ones for plane yi - 1, e_y = 0 ones for plane yi,
+//      e_y = +1 ones for plane yi + 1;
+//   3. the second step of plane yi - 1, whose 19 pulled values are now
+//      complete, stored to buf[p ^ 1].
+// The partial pulls of planes yi and yi + 1 stay in registers.
+//
+// Per node and step the arithmetic, the rounding of the stored value and its
+// seed are those of k_step, so two launches of k_step and one of k_step2 give
+// the same state bit for bit; the memory traffic of the pair is one read and
+// one write of the populations (plus the halo) instead of two of each.  The
+// intermediate state never reaches memory, so the launch before a read-back
+// must be a k_step (the export streams the state the last launch read).
+// ---------------------------------------------------------------------------
+#ifndef NFLOW_STEP2_TX
+#define NFLOW_STEP2_TX 256
+#define NFLOW_STEP2_TZ 8
+#define NFLOW_STEP2_NT 1024
+#endif
+namespace step2 {
+// Slot of crossing population q (e_x or e_z non-zero) in an LDS row: the 5
+// with e_z = -1 first (3, 7, 8, 15, 17), then the 6 with e_z = 0, then the 5
+// with e_z = +1 (4, 9, 10, 16, 18); -1: the own column keeps it (rest, e =
+// (0, +-1, 0)).  The rows of the tile are stored one after another, the halo
+// row z0 - 1 before them with only its e_z = +1 slots (the ones pulled into
+// the tile) and the halo row z0 + TZ after them with only its e_z = -1 slots,
+// so that every access is one per-thread base plus a constant.
+constexpr int CIX[Q] = {-1, 5, 6, 0, 11, -1, -1, 1, 2, 12, 13, 7, 8, 9, 10, 3, 14, 4, 15};
+constexpr int NC = 16;
+constexpr int QUP = 5, QDN = 6;  // e = (0, +1, 0), (0, -1, 0)
+
+template <int TX_, int TZ_, int NT_>
+struct Geo {
+  static constexpr int TX = TX_, TZ = TZ_, NT = NT_, WX = TX + 2;
+  static constexpr int RP = NT / TX;  // x rows per pass of the threads
+  static constexpr int K = TZ / RP;   // own nodes per thread
+  static_assert(NT % TX == 0 && TZ % RP == 0, "whole rows per thread pass");
+  static_assert(TX % 64 == 0, "a wave within one row group");
+  // LDS element of slot c of row zs (-1 .. TZ) at column-frame x xs
+  static constexpr int LN = (TZ * NC + 10) * WX;
+  __device__ static constexpr int lidx(int zs, int c, int xs) { return ((zs + 1) * NC + c - 11) * WX + xs; }
+};
+}  // namespace step2
+
+template <class St, class G>
+__global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H, int ntx, int ntz) {
+  static_assert(!St::shifted, "two steps per launch are built for FP32 storage");
+  using namespace step2;
+  using T = typename St::T;
+  constexpr int TX = G::TX, TZ = G::TZ, NT = G::NT, WX = G::WX, K = G::K;
+  // crossing populations of the own rows (with the x - 1 and x + TX columns)
+  // and of the halo rows z0 - 1 (e_z = +1 ones) and z0 + TZ (e_z = -1 ones)
+  __shared__ T cross[G::LN];
+  int bt = blockIdx.x;
+  // workgroups are dispatched to the 8 XCDs round-robin: give each XCD a
+  // contiguous range of tiles, so that neighbours share their halo in L2
+  if ((gridDim.x & 7u) == 0) bt = (bt & 7) * (int)(gridDim.x >> 3) + (bt >> 3);
+  const int tx = bt % ntx, tz = (bt / ntx) % ntz, ch = bt / (ntx * ntz);
+  const int x0 = tx * TX, z0 = tz * TZ, ya = ch * H, yb = min(L.ny, ya + H);
+  const int p = parity ^ L.parity0;
+  const GPops<T> in{static_cast<T*>(L.buf[p])}, out{static_cast<T*>(L.buf[p ^ 1])};
+  const int t = threadIdx.x;
+  // own nodes: (x0 + lx, z0 + zr0 + j), j < K; the row group is the same for
+  // a whole wave
+  const int lx = t % TX, zr0 = __builtin_amdgcn_readfirstlane(t / TX) * K;
+  const int ox = x0 + lx;
+  const bool xin = ox < L.nx, xlo = ox == 0, xhi = ox == L.nx - 1;
+  // halo nodes of the tile: row z0 - 1, row z0 + TZ (x from xa to xb), column
+  // x0 - 1, column x0 + TX (z0 .. z0 + tzn - 1), each where inside the box
+  const int xa = max(x0 - 1, 0), xb = min(x0 + TX, L.nx - 1), rowlen = xb - xa + 1;
+  const int tzn = min(TZ, L.nz - z0);
+  const int nlo = z0 > 0 ? rowlen : 0, nhi = z0 + TZ < L.nz ? rowlen : 0;
+  const int nxl = x0 > 0 ? tzn : 0, nxr = x0 + TX < L.nx ? tzn : 0;
+  const int nhalo = nlo + nhi + nxl + nxr;
+  auto node = [&](int x, int y, int z) {
+    return (uint32_t)x + (uint32_t)L.nx * ((uint32_t)z + (uint32_t)L.nz * (uint32_t)y);
+  };
+  // first step on node (x, yi, z): e = the stored values, g = them as the
+  // next step reads them back
+  auto step1 = [&](int x, int yi, int z, T e[Q], float g[Q]) {
+    const uint32_t n = node(x, yi, z);
+    pull<St, false>(L, in, n, n, x, yi, z, g);
+    boundary(L, SideIn{{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}}, x, yi, z, g);
+    relax<false>(L, g, nullptr, x, yi, z);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      e[q] = St::enc(g[q] - shift_of<St>(q));
+      g[q] = St::dec(e[q]) + shift_of<St>(q);
+    }
+  };
+
+  // Pulled values of own node j: fp on plane yi - 1 (complete but for the
+  // e_y = -1 crossing links from plane yi), fc on plane yi (so far its e_y =
+  // +1 links), fn the e_y = +1 links of plane yi + 1.
+  float fp[K][Q], fc[K][Q], fn[K][Q];
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fp[j][q] = fc[j][q] = fn[j][q] = 0.0f;
+  // does the link of q at own node j on plane y leave the box?
+  auto gone = [&](int q, int j, int y) {
+    const int oz = z0 + zr0 + j;
+    return (EX[q] > 0 && xlo) || (EX[q] < 0 && xhi) || (EY[q] > 0 && y == 0) || (EY[q] < 0 && y == L.ny - 1) ||
+           (EZ[q] > 0 && oz == 0) || (EZ[q] < 0 && oz == L.nz - 1);
+  };
+  for (int yi = ya - 1; yi <= yb; ++yi) {
+    const bool plane = yi >= 0 && yi < L.ny;
+    if (plane) {
+      for (int h = t; h < nhalo; h += NT) {  // halo
+        int x, z, kind;  // kind 0: row z0 - 1, 1: row z0 + TZ, 2: a column
+        if (h < nlo) { x = xa + h; z = z0 - 1; kind = 0; }
+        else if (h < nlo + nhi) { x = xa + (h - nlo); z = z0 + TZ; kind = 1; }
+        else if (h < nlo + nhi + nxl) { x = x0 - 1; z = z0 + (h - nlo - nhi); kind = 2; }
+        else { x = x0 + TX; z = z0 + (h - nlo - nhi - nxl); kind = 2; }
+        T e[Q];
+        float g[Q];
+        step1(x, yi, z, e, g);
+        if (kind == 2) {
+#pragma unroll
+          for (int q = 0; q < Q; ++q)
+            if (CIX[q] >= 0) cross[G::lidx(z - z0, CIX[q], x - x0 + 1)] = e[q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < Q; ++q)
+            if (EZ[q] != 0 && (EZ[q] > 0) == (kind == 0)) cross[G::lidx(z - z0, CIX[q], x - x0 + 1)] = e[q];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {  // own nodes
+      const int oz = z0 + zr0 + j;
+      float g[Q];
+      if (plane && xin && oz < L.nz) {
+        T e[Q];
+        step1(ox, yi, oz, e, g);
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+          if (CIX[q] >= 0) cross[G::lidx(zr0 + j, CIX[q], lx + 1)] = e[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) g[q] = 0.0f;
+      }
+      fc[j][0] = g[0];
+      fn[j][QUP] = g[QUP];
+      fp[j][QDN] = yi - 1 == L.ny - 1 ? fp[j][QDN] : g[QDN];  // leaves only from plane ny - 1
+      fc[j][QDN] = gone(QDN, j, yi) ? g[QUP] : 0.0f;
+      fc[j][QUP] = gone(QUP, j, yi) ? g[QDN] : fc[j][QUP];
+    }
+    __syncthreads();
+    // crossing population q pulled by own node j from plane yi
+    auto pulled = [&](int q, int j) {
+      return St::dec(cross[G::lidx(zr0 + j - EZ[q], CIX[q], lx + 1 - EX[q])]) + shift_of<St>(q);
+    };
+    // own node j's crossing population q after the first step on plane yi
+    auto own = [&](int q, int j) { return St::dec(cross[G::lidx(zr0 + j, CIX[q], lx + 1)]) + shift_of<St>(q); };
+    const int yf = yi - 1;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {  // second step on plane yf
+      const int oz = z0 + zr0 + j;
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (CIX[q] >= 0 && EY[q] < 0) fp[j][q] = gone(q, j, yf) ? fp[j][q] : pulled(q, j);
+      if (xin && oz < L.nz && yf >= ya && yf < yb) {
+        const uint32_t n = node(ox, yf, oz);
+        boundary(L, SideIn{{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}}, ox, yf, oz, fp[j]);
+        relax_and_store<St, false, false>(L, fp[j], out, nullptr, n, ox, yf, oz, 0u);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {  // what planes yi and yi + 1 pull from plane yi
+#pragma unroll
+      for (int q = 1; q < Q; ++q) {
+        if (CIX[q] < 0) continue;
+        const bool out_q = gone(q, j, yi);
+        const float b = out_q ? own(OPP[q], j) : 0.0f;
+        if (EY[q] < 0) fc[j][q] = b;
+        else if (EY[q] == 0) fc[j][q] = out_q ? b : pulled(q, j);
+        else {
+          fc[j][q] = out_q ? b : fc[j][q];
+          fn[j][q] = pulled(q, j);
+        }
+      }
+    }
+    __syncthreads();  // the next plane's first step overwrites LDS
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        fp[j][q] = fc[j][q];
+        if (EY[q] > 0) fc[j][q] = fn[j][q];
+      }
+  }
 }
 
 // Initialisation (Initialize vertex, lbm_codelets.cpp:140-241) or import of
@@ -843,6 +1090,7 @@ __global__ void __launch_bounds__(EXPORT_BS) k_export(LatticeDev L, int prev, in
   }
 }
 
+#if NFLOW_STORAGE_SET != 2  // the FP32 object carries it
 __global__ void __launch_bounds__(256) k_reduce(const double* __restrict__ partials, int nblocks,
                                                 double* __restrict__ out) {
   __shared__ double sh[256];
@@ -859,6 +1107,7 @@ __global__ void __launch_bounds__(256) k_reduce(const double* __restrict__ parti
     __syncthreads();
   }
 }
+#endif
 
 // Instability probe over a batch's work list: the rest population of every
 // node in the newest state (the buffer the next step reads).  A NaN or
@@ -881,12 +1130,15 @@ __global__ void __launch_bounds__(512) k_probe(const LatticeDev* __restrict__ la
   }
 }
 
+#if NFLOW_STORAGE_SET != 2  // the FP32 object carries it
 __global__ void __launch_bounds__(256) k_flags(const LatticeDev* __restrict__ lats, int nlat, uint32_t* __restrict__ out) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < nlat) out[i] = lats[i].ctr[CTR_DIVERGED];
 }
+#endif
 
 // keSetNodeType + keInitialNodeDirection (kernel.cu:1020-1097) for read-back.
+#if NFLOW_STORAGE_SET != 2  // the FP32 object carries it
 __global__ void __launch_bounds__(256) k_node_type(LatticeDev L, int* __restrict__ type, int* __restrict__ dirstate) {
   const uint32_t n = blockIdx.x * 256u + threadIdx.x;
   if (n >= L.n) return;
@@ -901,20 +1153,24 @@ __global__ void __launch_bounds__(256) k_node_type(LatticeDev L, int* __restrict
   type[n] = is_solid(L, n) ? NODE_STRUCT : node_type(L, x, y, z);
   dirstate[n] = ds;
 }
+#endif
 
 // Streaming-copy ceiling: `streams` arrays of `per` floats (per % 4 == 0),
 // each thread copying one 16-byte vector of every stream, as the step kernel
 // touches one element of every population plane.
+#if NFLOW_STORAGE_SET != 2  // the FP32 object carries it
 __global__ void __launch_bounds__(256) k_copy_streams(const float4* __restrict__ src, float4* __restrict__ dst,
                                                       size_t per4, int streams) {
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= per4) return;
   for (int s = 0; s < streams; ++s) dst[(size_t)s * per4 + i] = src[(size_t)s * per4 + i];
 }
+#endif
 
 // The same streaming ceiling in the FP32 population layout itself: every
 // thread moves one 4-byte element of each of the 19 runs of its 64-node block,
 // as k_step's own-node stores do, with the same non-temporal hints.
+#if NFLOW_STORAGE_SET != 2  // the FP32 object carries it
 __global__ void __launch_bounds__(256) k_copy_runs(const float* __restrict__ src, float* __restrict__ dst,
                                                    size_t nblk) {
   const size_t node = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -926,6 +1182,7 @@ __global__ void __launch_bounds__(256) k_copy_runs(const float* __restrict__ src
 #pragma unroll
   for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(v[q], (gptr<float>)dst + (blk * Q + q) * 64 + lane);
 }
+#endif
 
 // The 16-bit store conversion of the step kernel applied to an array:
 // to nearest (enc) or stochastically with the given seeds (enc_sr).
@@ -940,6 +1197,7 @@ __global__ void __launch_bounds__(256) k_round16(const float* __restrict__ src, 
 
 // VALU issue ceiling: eight independent v_fma_f32 chains per lane, 64 FMAs
 // per loop iteration, no memory traffic until the single store at the end.
+#if NFLOW_STORAGE_SET != 2  // the FP32 object carries it
 __global__ void __launch_bounds__(256) k_fma_chains(float* __restrict__ out, float a, float b, int iters) {
   float x[8];
 #pragma unroll
@@ -956,9 +1214,18 @@ __global__ void __launch_bounds__(256) k_fma_chains(float* __restrict__ out, flo
   for (int c = 0; c < 8; ++c) s += x[c];
   out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
 }
+#endif
 
 // ---------------------------------------------------------------------------
-// Launchers
+// Launchers.  This file is compiled twice.  NFLOW_STORAGE_SET 1 (the default)
+// builds the FP32 kernels, the kernels that do not depend on the storage
+// format and the launch entry points; it is compiled with -ffp-contract=on,
+// which the bitwise equality of one- and two-step launches needs (an FMA then
+// never spans statements, so the collision rounds the same in every kernel it
+// is inlined into).  NFLOW_STORAGE_SET 2 builds the FP16 / BF16 kernels,
+// reached through the *_h functions, with the compiler's default contraction
+// (fused across statements: 2.5% faster at FP16, and no two-step kernel there
+// to match).  Each template instantiation lives in exactly one of the two.
 // ---------------------------------------------------------------------------
 template <class St, bool MASK, bool SR>
 static hipError_t step_bs(int block, int grid, hipStream_t st, const LatticeDev* d_lats, const WorkItem* d_items,
@@ -988,30 +1255,33 @@ static hipError_t step_mask(bool masked, int block, int grid, hipStream_t st, co
                 : step_bs<St, false, SR>(block, grid, st, d_lats, d_items, nitems, parity);
 }
 
-hipError_t launch_step(int storage, bool masked, bool sr, int block, int grid, hipStream_t st,
-                       const LatticeDev* d_lats, int nlat, const WorkItem* d_items, int nitems, int parity) {
-  if (grid <= 0 || nitems <= 0) return hipSuccess;
-  hipError_t e = hipErrorInvalidValue;
+// 16-bit storage (NFLOW_STORAGE_SET 2)
+hipError_t launch_step_h(int storage, bool masked, bool sr, int block, int grid, hipStream_t st,
+                         const LatticeDev* d_lats, const WorkItem* d_items, int nitems, int parity);
+hipError_t launch_init_h(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src, uint32_t step0);
+hipError_t launch_export_h(int storage, hipStream_t st, const LatticeDev& L, int prev, int prev_streamed, float* aos,
+                           float* rho, float* ux, float* uy, float* uz, double* partials, int grid);
+hipError_t launch_probe_h(int storage, hipStream_t st, const LatticeDev* d_lats, const WorkItem* d_items, int nitems,
+                          int grid, int parity);
+
+#if NFLOW_STORAGE_SET == 2
+
+hipError_t launch_step_h(int storage, bool masked, bool sr, int block, int grid, hipStream_t st,
+                         const LatticeDev* d_lats, const WorkItem* d_items, int nitems, int parity) {
   switch (storage) {
-    case ST_F32: e = step_mask<StF32, false>(masked, block, grid, st, d_lats, d_items, nitems, parity); break;
     case ST_F16:
-      e = sr ? step_mask<StF16, true>(masked, block, grid, st, d_lats, d_items, nitems, parity)
-             : step_mask<StF16, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
-      break;
+      return sr ? step_mask<StF16, true>(masked, block, grid, st, d_lats, d_items, nitems, parity)
+                : step_mask<StF16, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
     case ST_BF16:
-      e = sr ? step_mask<StBF16, true>(masked, block, grid, st, d_lats, d_items, nitems, parity)
-             : step_mask<StBF16, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
-      break;
+      return sr ? step_mask<StBF16, true>(masked, block, grid, st, d_lats, d_items, nitems, parity)
+                : step_mask<StBF16, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
   }
-  if (e != hipSuccess || !sr || storage == ST_F32) return e;
-  hipLaunchKernelGGL(k_advance, dim3((unsigned)((nlat + 255) / 256)), dim3(256), 0, st, d_lats, nlat, parity);
-  return hipGetLastError();
+  return hipErrorInvalidValue;
 }
 
-hipError_t launch_init(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src, uint32_t step0) {
+hipError_t launch_init_h(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src, uint32_t step0) {
   const int grid = (int)((L.n + 255u) / 256u);
   switch (storage) {
-    case ST_F32: hipLaunchKernelGGL(k_init<StF32>, dim3(grid), dim3(256), 0, st, L, cur, src, step0); break;
     case ST_F16: hipLaunchKernelGGL(k_init<StF16>, dim3(grid), dim3(256), 0, st, L, cur, src, step0); break;
     case ST_BF16: hipLaunchKernelGGL(k_init<StBF16>, dim3(grid), dim3(256), 0, st, L, cur, src, step0); break;
     default: return hipErrorInvalidValue;
@@ -1019,17 +1289,9 @@ hipError_t launch_init(int storage, hipStream_t st, const LatticeDev& L, int cur
   return hipGetLastError();
 }
 
-int export_block_size() { return EXPORT_BS; }
-
-hipError_t launch_export(int storage, hipStream_t st, const LatticeDev& L, int prev, int prev_streamed, float* aos,
-                         float* rho, float* ux, float* uy, float* uz, double* partials, int* nblocks_out) {
-  const int grid = (int)((L.n + EXPORT_BS - 1) / EXPORT_BS);
-  if (nblocks_out) *nblocks_out = grid;
+hipError_t launch_export_h(int storage, hipStream_t st, const LatticeDev& L, int prev, int prev_streamed, float* aos,
+                           float* rho, float* ux, float* uy, float* uz, double* partials, int grid) {
   switch (storage) {
-    case ST_F32:
-      hipLaunchKernelGGL(k_export<StF32>, dim3(grid), dim3(EXPORT_BS), 0, st, L, prev, prev_streamed, aos, rho, ux, uy,
-                         uz, partials);
-      break;
     case ST_F16:
       hipLaunchKernelGGL(k_export<StF16>, dim3(grid), dim3(EXPORT_BS), 0, st, L, prev, prev_streamed, aos, rho, ux, uy,
                          uz, partials);
@@ -1043,6 +1305,73 @@ hipError_t launch_export(int storage, hipStream_t st, const LatticeDev& L, int p
   return hipGetLastError();
 }
 
+hipError_t launch_probe_h(int storage, hipStream_t st, const LatticeDev* d_lats, const WorkItem* d_items, int nitems,
+                          int grid, int parity) {
+  switch (storage) {
+    case ST_F16: hipLaunchKernelGGL(k_probe<StF16>, dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity); break;
+    case ST_BF16:
+      hipLaunchKernelGGL(k_probe<StBF16>, dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_round16(hipStream_t st, int storage, const float* src, const uint32_t* seeds, uint16_t* dst, size_t n,
+                          int stochastic) {
+  const size_t blocks = (n + 255) / 256;
+  if (storage == ST_F16)
+    hipLaunchKernelGGL(k_round16<StF16>, dim3((unsigned)blocks), dim3(256), 0, st, src, seeds, dst, n, stochastic);
+  else if (storage == ST_BF16)
+    hipLaunchKernelGGL(k_round16<StBF16>, dim3((unsigned)blocks), dim3(256), 0, st, src, seeds, dst, n, stochastic);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+#else  // NFLOW_STORAGE_SET 1
+
+hipError_t launch_step(int storage, bool masked, bool sr, int block, int grid, hipStream_t st,
+                       const LatticeDev* d_lats, int nlat, const WorkItem* d_items, int nitems, int parity) {
+  if (grid <= 0 || nitems <= 0) return hipSuccess;
+  if (storage != ST_F32) {
+    const hipError_t e = launch_step_h(storage, masked, sr, block, grid, st, d_lats, d_items, nitems, parity);
+    if (e != hipSuccess || !sr) return e;
+    hipLaunchKernelGGL(k_advance, dim3((unsigned)((nlat + 255) / 256)), dim3(256), 0, st, d_lats, nlat, parity);
+    return hipGetLastError();
+  }
+  return step_mask<StF32, false>(masked, block, grid, st, d_lats, d_items, nitems, parity);
+}
+
+hipError_t launch_step2(int storage, hipStream_t st, const LatticeDev& L, int parity, int H) {
+  using G = step2::Geo<NFLOW_STEP2_TX, NFLOW_STEP2_TZ, NFLOW_STEP2_NT>;
+  if (H < 1 || storage != ST_F32) return hipErrorInvalidValue;
+  const int ntx = (L.nx + G::TX - 1) / G::TX, ntz = (L.nz + G::TZ - 1) / G::TZ, nch = (L.ny + H - 1) / H;
+  hipLaunchKernelGGL((k_step2<StF32, G>), dim3((unsigned)(ntx * ntz * nch)), dim3(G::NT), 0, st, L, parity, H, ntx,
+                     ntz);
+  return hipGetLastError();
+}
+
+hipError_t launch_init(int storage, hipStream_t st, const LatticeDev& L, int cur, const float* src, uint32_t step0) {
+  if (storage != ST_F32) return launch_init_h(storage, st, L, cur, src, step0);
+  const int grid = (int)((L.n + 255u) / 256u);
+  hipLaunchKernelGGL(k_init<StF32>, dim3(grid), dim3(256), 0, st, L, cur, src, step0);
+  return hipGetLastError();
+}
+
+int export_block_size() { return EXPORT_BS; }
+
+hipError_t launch_export(int storage, hipStream_t st, const LatticeDev& L, int prev, int prev_streamed, float* aos,
+                         float* rho, float* ux, float* uy, float* uz, double* partials, int* nblocks_out) {
+  const int grid = (int)((L.n + EXPORT_BS - 1) / EXPORT_BS);
+  if (nblocks_out) *nblocks_out = grid;
+  if (storage != ST_F32)
+    return launch_export_h(storage, st, L, prev, prev_streamed, aos, rho, ux, uy, uz, partials, grid);
+  hipLaunchKernelGGL(k_export<StF32>, dim3(grid), dim3(EXPORT_BS), 0, st, L, prev, prev_streamed, aos, rho, ux, uy, uz,
+                     partials);
+  return hipGetLastError();
+}
+
 hipError_t launch_reduce(hipStream_t st, const double* partials, int nblocks, double* out5) {
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, partials, nblocks, out5);
   return hipGetLastError();
@@ -1052,14 +1381,8 @@ hipError_t launch_probe(int storage, hipStream_t st, const LatticeDev* d_lats, c
                         int parity) {
   if (nitems <= 0) return hipSuccess;
   const int grid = nitems < 65536 ? nitems : 65536;
-  switch (storage) {
-    case ST_F32: hipLaunchKernelGGL(k_probe<StF32>, dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity); break;
-    case ST_F16: hipLaunchKernelGGL(k_probe<StF16>, dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity); break;
-    case ST_BF16:
-      hipLaunchKernelGGL(k_probe<StBF16>, dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity);
-      break;
-    default: return hipErrorInvalidValue;
-  }
+  if (storage != ST_F32) return launch_probe_h(storage, st, d_lats, d_items, nitems, grid, parity);
+  hipLaunchKernelGGL(k_probe<StF32>, dim3(grid), dim3(512), 0, st, d_lats, d_items, nitems, parity);
   return hipGetLastError();
 }
 
@@ -1081,18 +1404,6 @@ hipError_t launch_copy_runs(hipStream_t st, const float* src, float* dst, size_t
   return hipGetLastError();
 }
 
-hipError_t launch_round16(hipStream_t st, int storage, const float* src, const uint32_t* seeds, uint16_t* dst, size_t n,
-                          int stochastic) {
-  const size_t blocks = (n + 255) / 256;
-  if (storage == ST_F16)
-    hipLaunchKernelGGL(k_round16<StF16>, dim3((unsigned)blocks), dim3(256), 0, st, src, seeds, dst, n, stochastic);
-  else if (storage == ST_BF16)
-    hipLaunchKernelGGL(k_round16<StBF16>, dim3((unsigned)blocks), dim3(256), 0, st, src, seeds, dst, n, stochastic);
-  else
-    return hipErrorInvalidValue;
-  return hipGetLastError();
-}
-
 hipError_t launch_fma_chains(hipStream_t st, float* out, int grid, int iters) {
   hipLaunchKernelGGL(k_fma_chains, dim3((unsigned)grid), dim3(256), 0, st, out, 0.999f, 0.001f, iters);
   return hipGetLastError();
@@ -1105,5 +1416,7 @@ hipError_t launch_copy_streams(hipStream_t st, const float* src, float* dst, siz
                      reinterpret_cast<float4*>(dst), per4, streams);
   return hipGetLastError();
 }
+
+#endif  // NFLOW_STORAGE_SET
 
 }  // namespace nflow

@@ -255,6 +255,7 @@ int Batch::set_tuning(const nflow_tuning& t) {
     return fail(NFLOW_EINVAL, "tuning.block must be 64, 128, 256 or 512");
   if (t.persistent < 0 || t.graph_steps < 0 || (t.graph_steps & 1))
     return fail(NFLOW_EINVAL, "tuning.persistent must be >= 0 and graph_steps even and >= 0");
+  if (t.temporal < 0) return fail(NFLOW_EINVAL, "tuning.temporal must be >= 0");
   tuning = t;
   tables_dirty = true;
   destroy_graphs();
@@ -383,6 +384,19 @@ int Batch::launch_steps(int nsteps) {
   int rc = upload_tables();
   if (rc) return rc;
   int left = nsteps;
+  // two steps per launch: one FP32 cavity lattice without solid nodes; the
+  // last launch stays a single step so that the read-back finds the state it
+  // streams (the intermediate state of a pair never reaches memory)
+  if (tuning.temporal > 0 && storage == NFLOW_STORAGE_F32 && lats.size() == 1 && lats[0].in.casetype == NFLOW_CAVITY &&
+      !lats[0].dev.mask && left >= 3) {
+    const int pairs = (left - 1) / 2;
+    for (int k = 0; k < pairs; ++k) {
+      NFLOW_HIP(launch_step2(storage, stream, lats[0].dev, gparity, tuning.temporal));
+      gparity ^= 1;
+      ++launches;
+    }
+    left -= 2 * pairs;
+  }
   const int gs = tuning.graph_steps;
   if (gs > 0 && left >= gs && nitems > 0) {
     if (graph_steps_built != gs) destroy_graphs();

@@ -43,7 +43,7 @@ class Input(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("xcd_order", ctypes.c_int), ("persistent", ctypes.c_int),
-                ("graph_steps", ctypes.c_int), ("ms_per_step", ctypes.c_double)]
+                ("graph_steps", ctypes.c_int), ("temporal", ctypes.c_int), ("ms_per_step", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -359,8 +359,8 @@ class Batch:
         _check(lib().nflow_batch_autotune(self.h, steps_per_trial, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, block=256, xcd_order=0, persistent=0, graph_steps=10):
-        t = Tuning(block, xcd_order, persistent, graph_steps, 0.0)
+    def set_tuning(self, block=256, xcd_order=0, persistent=0, graph_steps=10, temporal=0):
+        t = Tuning(block, xcd_order, persistent, graph_steps, temporal, 0.0)
         _check(lib().nflow_batch_set_tuning(self.h, ctypes.byref(t)))
 
     def get_tuning(self):

@@ -1,0 +1,83 @@
+"""Two time steps per launch (nflow_tuning.temporal, k_step2; FP32 cavity
+lattices) against one step per launch: the same state bit for bit, for both
+collision models, partial tiles and y chunks, several x tiles, tiny boxes,
+the benchmark size, split runs, and a read-back after every run (the last
+launch of a run is a single step, so the export finds the state it
+streams)."""
+import numpy as np
+import pytest
+
+import nflow
+from helpers import cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def run(c, storage, sr, steps_list, temporal):
+    inp = nflow.defaults(**c, storage=storage)
+    inp.stochastic_rounding = sr
+    b = nflow.Batch([inp])
+    b.set_tuning(256, 0, 0, 0, temporal)
+    b.initialize()
+    out = []
+    for n in steps_list:
+        b.run(n)
+        out.append((b.read_arF(0), b.read_macro(0), b.diagnostics(0)))
+    b.close()
+    return out
+
+
+CASES = [
+    cfg(nx=64, ny=48, nz=40, enable_les=1, smagorinsky=0.17, tau=0.56, init_lid_velocity=1),
+    cfg(nx=37, ny=29, nz=23, enable_mrt=0, tau=0.6, init_lid_velocity=1),
+    cfg(nx=70, ny=5, nz=17, enable_les=1, smagorinsky=0.17, tau=0.53),
+    cfg(nx=3, ny=2, nz=5, enable_les=1, smagorinsky=0.17, tau=0.7, init_lid_velocity=1),
+    # wider than one x tile: halo columns between tiles, a partial last tile
+    cfg(nx=300, ny=11, nz=20, enable_les=1, smagorinsky=0.17, tau=0.55, init_lid_velocity=1),
+]
+
+
+@pytest.mark.parametrize("k", range(len(CASES)))
+def test_two_steps_per_launch_equal_one(k):
+    c = CASES[k]
+    steps = [7, 8, 3, 20]
+    ref = run(c, nflow.F32, 0, steps, 0)
+    for h in (1, 3, 32):
+        got = run(c, nflow.F32, 0, steps, h)
+        for j, ((a, m, d), (a0, m0, d0)) in enumerate(zip(got, ref)):
+            assert np.array_equal(a, a0), (h, j)
+            for x, y in zip(m, m0):
+                assert np.array_equal(x, y), (h, j)
+            assert d[0] == d0[0] and np.array_equal(d[1], d0[1]) and d[2] == d0[2], (h, j)
+
+
+def test_two_steps_per_launch_at_256_cubed():
+    c = cfg(nx=256, ny=256, nz=256, enable_les=1, smagorinsky=0.17, tau=0.6)
+    out = []
+    for h in (0, 32):
+        b = nflow.Batch([nflow.defaults(**c, storage=nflow.F32)])
+        b.set_tuning(512, 0, 0, 0, h)
+        b.initialize()
+        b.run(9)
+        out.append((b.read_macro(0), b.diagnostics(0)))
+        b.close()
+    (m, d), (m2, d2) = out
+    for x, y in zip(m, m2):
+        assert np.array_equal(x, y)
+    assert d[0] == d2[0] and np.array_equal(d[1], d2[1]) and d[2] == d2[2]
+
+
+@pytest.mark.parametrize("storage,sr", [(nflow.F16, 1), (nflow.BF16, 0)])
+def test_temporal_is_ignored_for_16_bit_storage(storage, sr):
+    a = run(CASES[0], storage, sr, [9], 0)[0][0]
+    b = run(CASES[0], storage, sr, [9], 16)[0][0]
+    assert np.array_equal(a, b)
+
+
+def test_temporal_is_ignored_where_it_does_not_apply():
+    # a channel lattice keeps one step per launch
+    ch = cfg(nx=40, ny=12, nz=10, casetype=nflow.CHANNEL, enable_les=1, smagorinsky=0.17, tau=0.56,
+             inlet_peak=0.05, perturb=0.01, seed=3)
+    a = run(ch, nflow.F32, 0, [9], 0)[0][0]
+    b = run(ch, nflow.F32, 0, [9], 16)[0][0]
+    assert np.array_equal(a, b)
