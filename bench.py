@@ -67,9 +67,9 @@ def make_input(nflow, w):
     return p
 
 
-def traffic_from_profiles(name):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if recorded."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+def traffic_from_profiles(name, kernel="k_step"):
+    """HBM bytes per launch of the kernel measured by rocprofv3 --pmc, if recorded."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}{'_step2' if kernel == 'k_step2' else ''}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -77,15 +77,16 @@ def traffic_from_profiles(name):
     return d.get("bytes_per_launch")
 
 
-def valu_per_node_from_profiles(name):
-    """k_step VALU instructions per node (SQ_INSTS_VALU / SQ_WAVES / 64 lanes,
-    one node per lane) measured by rocprofv3 --pmc, if recorded."""
-    path = os.path.join(ROOT, "profiles", f"instructions_{name}.json")
+def valu_per_node_from_profiles(name, kernel="k_step"):
+    """VALU instructions of one launch of the kernel per 64 nodes of the
+    lattice (SQ_INSTS_VALU x 64 / nodes) measured by rocprofv3 --pmc, if
+    recorded."""
+    path = os.path.join(ROOT, "profiles", f"instructions_{name}{'_step2' if kernel == 'k_step2' else ''}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get("k_step", {}).get("valu_per_node")
+    return d.get(kernel, {}).get("valu_per_node")
 
 
 def accuracy_leg(nflow, w):
@@ -468,8 +469,9 @@ def main():
     ap.add_argument("--serve-seed", type=int, default=7, help="served-stream leg: seed")
     ap.add_argument("--no-autotune", action="store_true", help="fixed launch shape instead of the autotuner's")
     ap.add_argument("--graph-steps", type=int, default=10)
-    ap.add_argument("--temporal", type=int, default=0,
-                    help="two steps per launch, y swept in chunks of this many planes (0: one step per launch)")
+    ap.add_argument("--temporal", type=int, default=None,
+                    help="FP32 cavity: two steps per launch, y swept in chunks of this many planes (0: one step per "
+                         "launch, -1: the library's choice; default: the autotuner's)"),
     ap.add_argument("--block", type=int, default=512, help="workgroup size with --no-autotune")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--requests", type=int, default=48, help="serve mode: length of the request stream")
@@ -504,9 +506,11 @@ def main():
     if not args.no_autotune:
         tuning = b.autotune(steps_per_trial=10)
         tuned_ms = tuning["ms_per_step"]
-        b.set_tuning(tuning["block"], tuning["xcd_order"], tuning["persistent"], args.graph_steps, args.temporal)
+        b.set_tuning(tuning["block"], tuning["xcd_order"], tuning["persistent"], args.graph_steps,
+                     tuning["temporal"] if args.temporal is None else args.temporal)
     else:
-        b.set_tuning(args.block, 0, 0, args.graph_steps, args.temporal)
+        b.set_tuning(args.block, 0, 0, args.graph_steps, -1 if args.temporal is None else args.temporal)
+    h2 = b.step2_height()
     b.initialize()
     b.run(args.warmup)
     b.synchronize()
@@ -523,22 +527,41 @@ def main():
     b.synchronize()
     wall = time.perf_counter() - t0
     dev_ms = statistics.median(seg_ms)
-    m, mom, _ = b.diagnostics(0)
     n = w["nx"] * w["ny"] * w["nz"]
     mlups = n * args.steps / wall / 1e6
     bpu = BYTES_PER_UPDATE[w["storage"]]
-    achieved = bpu * n / (dev_ms * 1e-3) / 1e9
+    if h2 > 0:
+        # the dominant kernel is k_step2 (two steps per launch, one read and one
+        # write of the populations); its launch time from two runs of 2m + 1
+        # steps (m two-step launches and one single step each)
+        seg2 = []
+        for _ in range(3):
+            tot = []
+            for steps in (201, 401):
+                b.run(steps)
+                tot.append(b.timing()[0] * steps)
+            seg2.append((tot[1] - tot[0]) / 100)
+        kernel = {"name": "k_step2", "steps_per_launch": 2, "launch_ms": statistics.median(seg2),
+                  "launch_ms_samples": [round(x, 5) for x in seg2]}
+    else:
+        kernel = {"name": "k_step", "steps_per_launch": 1, "launch_ms": dev_ms}
+    m, mom, _ = b.diagnostics(0)
+    # algorithmic bytes per launch: one read and one write of the populations
+    # (bpu per node), whether the launch advances one step or two
+    bpl = bpu * n
+    achieved = bpl / (kernel["launch_ms"] * 1e-3) / 1e9
     # streaming ceilings of the same access width: one array, and 19 arrays
     # side by side (the population streams of a plane-per-population layout)
     copy1_gbs, _ = nflow.copy_bandwidth(2 * 19 * 4 * 4_000_000, streams=1, reps=20)
     copy_gbs, _ = nflow.copy_bandwidth(2 * 19 * 4 * 4_000_000, streams=19, reps=20)
     copy_runs_gbs, _ = nflow.copy_bandwidth(2 * 19 * 4 * 4_000_000, streams=0, reps=20)
     fma_g, _ = nflow.valu_rate(reps=5)
-    vpn = valu_per_node_from_profiles(args.workload)
-    valu = {"fma_chain_g_wave_per_s": round(fma_g, 1), "k_step_valu_per_node": vpn,
-            "k_step_g_wave_per_s": None if vpn is None else round(vpn * n / 64 / (dev_ms * 1e-3) / 1e9, 1)}
+    kn = kernel["name"]
+    vpn = valu_per_node_from_profiles(args.workload, kn)
+    valu = {"fma_chain_g_wave_per_s": round(fma_g, 1), "kernel": kn, "valu_per_64_nodes": vpn,
+            "g_wave_per_s": None if vpn is None else round(vpn * n / 64 / (kernel["launch_ms"] * 1e-3) / 1e9, 1)}
     if vpn is not None:
-        valu["k_step_over_fma_chain"] = round(valu["k_step_g_wave_per_s"] / fma_g, 3)
+        valu["over_fma_chain"] = round(valu["g_wave_per_s"] / fma_g, 3)
     out = {
         "metric": METRIC,
         "value": round(mlups, 1),
@@ -554,13 +577,15 @@ def main():
                                         ", stochastic rounding" if w.get("sr", 1) else ", rounded to nearest"),
         "data": "synthetic (lattice initialised at rest, rho = 1)",
         "config": {"workload": w["desc"], "nx": w["nx"], "ny": w["ny"], "nz": w["nz"], "nodes": n,
-                   "tuning": dict(b.get_tuning(), ms_per_step=tuned_ms, autotuned=not args.no_autotune,
-                                  graph_replay=args.graph_steps > 0 and args.steps // 5 >= args.graph_steps),
+                   "tuning": dict(b.get_tuning(), ms_per_step=tuned_ms, autotuned=not args.no_autotune, step2_height=h2,
+                                  graph_replay=h2 == 0 and args.graph_steps > 0 and args.steps // 5 >= args.graph_steps),
                    "device": name, "cus": cus},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(args.workload),
-                     "bytes_per_update": bpu, "kernel_ms": round(dev_ms, 5),
-                     "kernel_ms_segments": [round(x, 5) for x in seg_ms]},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(args.workload, kn),
+                     "kernel": kn, "steps_per_launch": kernel["steps_per_launch"], "bytes_per_launch": bpl,
+                     "bytes_per_update": bpu // kernel["steps_per_launch"], "kernel_ms": round(kernel["launch_ms"], 5),
+                     "kernel_ms_samples": kernel.get("launch_ms_samples"),
+                     "step_ms_segments": [round(x, 5) for x in seg_ms]},
         "hbm_frac_of_peak": round(achieved / HBM_PEAK_GBS, 4),
         "copy_ceiling_gbs": {"1_stream": round(copy1_gbs, 1), "19_streams": round(copy_gbs, 1),
                              "population_runs": round(copy_runs_gbs, 1)},

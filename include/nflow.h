@@ -204,11 +204,13 @@ typedef struct nflow_tuning {
   int xcd_order;   /* 1: each XCD receives a contiguous range of work items */
   int persistent;  /* 0: one workgroup per work item; k > 0: k workgroups per CU, grid-stride */
   int graph_steps; /* steps per captured graph (even), 0 = no graph */
-  /* H > 0: a batch of one FP32 cavity lattice without solid nodes advances two
-   * steps per launch (x-z tiles of 256 x 8 nodes swept along y in chunks of H
-   * planes; the last launch of each run stays a single step), with results bit
-   * for bit those of one step per launch; other batches ignore it.
-   * 0 = one step per launch */
+  /* Two steps per launch for a batch of one FP32 cavity lattice without solid
+   * nodes (other batches ignore this): x-z tiles of 256 x 8 nodes swept along
+   * y in chunks of H planes, the last launch of each run a single step, with
+   * results bit for bit those of one step per launch.  H > 0: that chunk
+   * height; -1 (the default): chosen by the library, on when nx is a multiple
+   * of 256 and a height of 16 to 64 planes fills whole rounds of one
+   * workgroup per CU; 0: one step per launch.  The autotuner times it. */
   int temporal;
   double ms_per_step; /* filled by the autotuner */
 } nflow_tuning;
@@ -218,6 +220,9 @@ int nflow_batch_initialize(nflow_batch* b);
 int nflow_batch_autotune(nflow_batch* b, int steps_per_trial, nflow_tuning* chosen);
 int nflow_batch_set_tuning(nflow_batch* b, const nflow_tuning* t);
 int nflow_batch_get_tuning(nflow_batch* b, nflow_tuning* t);
+/* chunk height the batch's runs use for two steps per launch under its tuning
+ * (temporal = -1 resolved), 0 when every launch is a single step */
+int nflow_batch_step2_height(nflow_batch* b);
 int nflow_batch_run(nflow_batch* b, int nsteps);
 int nflow_batch_synchronize(nflow_batch* b);
 int nflow_batch_read_arF(nflow_batch* b, int idx, float* dst);

@@ -48,7 +48,7 @@ class Batch {
   hipStream_t stream = nullptr;
   int device = 0;  // the device current at open(), which owns the stream
   int cu_count = 256;
-  nflow_tuning tuning{256, 0, 0, 10, 0, 0.0};
+  nflow_tuning tuning{256, 0, 0, 10, -1, 0.0};
 
   // device tables
   LatticeDev* d_lats = nullptr;
@@ -109,6 +109,9 @@ class Batch {
   int node_types(int idx, int* type, int* dirstate);
   int autotune(int trial_steps, nflow_tuning* out);
   int set_tuning(const nflow_tuning& t);
+  // y-chunk height of two-step launches for this batch under `t` (tuning by
+  // default), resolving temporal = -1; 0: one step per launch
+  int step2_height(const nflow_tuning* t = nullptr) const;
   void destroy_graphs();
   long long total_nodes() const;
   size_t device_bytes() const;

@@ -282,27 +282,6 @@ __device__ inline void pull(const LatticeDev& L, const Acc& in, uint32_t n, uint
   }
 }
 
-// pull() for a lattice without solid nodes, without the conversion to float:
-// the 19 stored values, so that a caller can issue the loads well ahead of
-// their use (the same elements as pull<St, false>(L, in, n, n, ...)).
-template <class St, class Acc>
-__device__ inline void pull_raw(const LatticeDev& L, const Acc& in, uint32_t n, int x, int y, int z,
-                                typename St::T r[Q]) {
-  const bool xlo = x == 0, xhi = x == L.nx - 1;
-  const bool ylo = y == 0, yhi = y == L.ny - 1;
-  const bool zlo = z == 0, zhi = z == L.nz - 1;
-  const uint32_t ps = (uint32_t)L.pstride;
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const bool out = (EX[q] > 0 && xlo) || (EX[q] < 0 && xhi) || (EY[q] > 0 && ylo) || (EY[q] < 0 && yhi) ||
-                     (EZ[q] > 0 && zlo) || (EZ[q] < 0 && zhi);
-    const int off = EX[q] + EZ[q] * L.nx + EY[q] * L.nxz;
-    const uint32_t idx =
-        out ? pidx<typename St::T>(OPP[q], n, ps) : pidx<typename St::T>(q, (uint32_t)((int)n - off), ps);
-    r[q] = in.get(idx);
-  }
-}
-
 // Inputs of the inlet / outlet rules from the previous step's side planes:
 // inlet, v[0] = rho(x+1) and v[1] = u_in; outlet, v[0..2] = u(nx-1) and
 // v[3..5] = u(nx-2).  Loaded before the 19 pulls, so that a wave holding an
@@ -752,11 +731,6 @@ __global__ void __launch_bounds__(256) k_advance(const LatticeDev* __restrict__ 
 // intermediate state never reaches memory, so the launch before a read-back
 // must be a k_step (the export streams the state the last launch read).
 // ---------------------------------------------------------------------------
-#ifndef NFLOW_STEP2_TX
-#define NFLOW_STEP2_TX 256
-#define NFLOW_STEP2_TZ 8
-#define NFLOW_STEP2_NT 1024
-#endif
 namespace step2 {
 // Slot of crossing population q (e_x or e_z non-zero) in an LDS row: the 5
 // with e_z = -1 first (3, 7, 8, 15, 17), then the 6 with e_z = 0, then the 5
@@ -1344,7 +1318,7 @@ hipError_t launch_step(int storage, bool masked, bool sr, int block, int grid, h
 }
 
 hipError_t launch_step2(int storage, hipStream_t st, const LatticeDev& L, int parity, int H) {
-  using G = step2::Geo<NFLOW_STEP2_TX, NFLOW_STEP2_TZ, NFLOW_STEP2_NT>;
+  using G = step2::Geo<STEP2_TX, STEP2_TZ, STEP2_NT>;
   if (H < 1 || storage != ST_F32) return hipErrorInvalidValue;
   const int ntx = (L.nx + G::TX - 1) / G::TX, ntz = (L.nz + G::TZ - 1) / G::TZ, nch = (L.ny + H - 1) / H;
   hipLaunchKernelGGL((k_step2<StF32, G>), dim3((unsigned)(ntx * ntz * nch)), dim3(G::NT), 0, st, L, parity, H, ntx,

@@ -31,6 +31,9 @@ enum : int { SIDE_RHO = 0, SIDE_UIN = 1, SIDE_U0 = 4, SIDE_U1 = 8, SIDE_REC = 12
 // density is not a positive finite number and by the probe when some node's
 // rest population is not finite; cleared only by initialisation.
 enum : int { CTR_DIVERGED = 2, CTR_WORDS = 4 };
+// Two steps per launch (k_step2): x-z tiles of STEP2_TX x STEP2_TZ nodes,
+// STEP2_NT threads per workgroup (two nodes each).
+enum : int { STEP2_TX = 256, STEP2_TZ = 8, STEP2_NT = 1024 };
 
 // Exact unsigned division by a runtime constant d for n < 2^31:
 // floor(n / d) = (n * magic) >> shift with shift = 31 + ceil(log2 d) and

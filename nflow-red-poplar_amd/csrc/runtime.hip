@@ -255,7 +255,7 @@ int Batch::set_tuning(const nflow_tuning& t) {
     return fail(NFLOW_EINVAL, "tuning.block must be 64, 128, 256 or 512");
   if (t.persistent < 0 || t.graph_steps < 0 || (t.graph_steps & 1))
     return fail(NFLOW_EINVAL, "tuning.persistent must be >= 0 and graph_steps even and >= 0");
-  if (t.temporal < 0) return fail(NFLOW_EINVAL, "tuning.temporal must be >= 0");
+  if (t.temporal < -1) return fail(NFLOW_EINVAL, "tuning.temporal must be >= -1");
   tuning = t;
   tables_dirty = true;
   destroy_graphs();
@@ -387,11 +387,11 @@ int Batch::launch_steps(int nsteps) {
   // two steps per launch: one FP32 cavity lattice without solid nodes; the
   // last launch stays a single step so that the read-back finds the state it
   // streams (the intermediate state of a pair never reaches memory)
-  if (tuning.temporal > 0 && storage == NFLOW_STORAGE_F32 && lats.size() == 1 && lats[0].in.casetype == NFLOW_CAVITY &&
-      !lats[0].dev.mask && left >= 3) {
+  const int h2 = step2_height();
+  if (h2 > 0 && left >= 3) {
     const int pairs = (left - 1) / 2;
     for (int k = 0; k < pairs; ++k) {
-      NFLOW_HIP(launch_step2(storage, stream, lats[0].dev, gparity, tuning.temporal));
+      NFLOW_HIP(launch_step2(storage, stream, lats[0].dev, gparity, h2));
       gparity ^= 1;
       ++launches;
     }
@@ -585,6 +585,27 @@ int Batch::node_types(int idx, int* type, int* dirstate) {
 // twice, in opposite orders, each keeping its better time.  The trial steps
 // advance the state, so every lattice is re-initialised afterwards (call
 // before importing a state).
+int Batch::step2_height(const nflow_tuning* t) const {
+  const nflow_tuning& tu = t ? *t : tuning;
+  if (tu.temporal == 0 || storage != NFLOW_STORAGE_F32 || lats.size() != 1) return 0;
+  const HostLattice& h = lats[0];
+  if (h.in.casetype != NFLOW_CAVITY || h.dev.mask) return 0;
+  if (tu.temporal > 0) return tu.temporal;
+  // automatic: whole tile rows in x (no idle lanes), and the fewest y chunks
+  // of 16 to 64 planes whose tiles fill whole rounds of one workgroup per CU
+  // (measured at 256^3, 256 x 128 x 256 and 512 x 256 x 512: 1.31-1.43x the
+  // one-step rate; with idle lanes or a partial last round it can be slower)
+  if (h.in.nx % STEP2_TX != 0) return 0;
+  const long cols = (long)(h.in.nx / STEP2_TX) * ((h.in.nz + STEP2_TZ - 1) / STEP2_TZ);
+  for (int nch = 1; nch <= h.in.ny; ++nch) {
+    const int hh = (h.in.ny + nch - 1) / nch;
+    if (hh > 64) continue;
+    if (hh < 16) break;
+    if ((cols * nch) % cu_count == 0) return hh;
+  }
+  return 0;
+}
+
 int Batch::autotune(int trial_steps, nflow_tuning* out) {
   if (trial_steps < 2) trial_steps = 2;
   const nflow_tuning base = tuning;
@@ -597,6 +618,7 @@ int Batch::autotune(int trial_steps, nflow_tuning* out) {
         t.xcd_order = xo;
         t.persistent = pe;
         t.graph_steps = 0;
+        t.temporal = 0;
         t.ms_per_step = 1e30;
         cand.push_back(t);
       }
@@ -621,6 +643,30 @@ int Batch::autotune(int trial_steps, nflow_tuning* out) {
   nflow_tuning best = cand[0];
   for (const auto& t : cand)
     if (t.ms_per_step < best.ms_per_step) best = t;
+  // two steps per launch (where the batch allows it): the automatic chunk
+  // height and a few fixed ones, each timed over pairs with the best shape
+  nflow_tuning probe = best;
+  probe.temporal = 16;
+  if (step2_height(&probe) > 0) {
+    probe.temporal = -1;
+    std::vector<int> hs = {16, 32, 64};
+    const int ah = step2_height(&probe);
+    if (ah > 0 && ah != 16 && ah != 32 && ah != 64) hs.push_back(ah);
+    std::vector<nflow_tuning> tc;
+    for (int hh : hs) {
+      nflow_tuning t = best;
+      t.temporal = hh;
+      t.ms_per_step = 1e30;
+      tc.push_back(t);
+    }
+    for (int pass = 0; pass < 2; ++pass)
+      for (size_t k = 0; k < tc.size(); ++k) {
+        rc = trial(tc[pass == 0 ? k : tc.size() - 1 - k], 4 * trial_steps + 1);
+        if (rc) return rc;
+      }
+    for (const auto& t : tc)
+      if (t.ms_per_step < best.ms_per_step) best = t;
+  }
   best.graph_steps = base.graph_steps;
   rc = set_tuning(best);
   if (rc) return rc;
@@ -973,6 +1019,11 @@ extern "C" int nflow_batch_get_tuning(nflow_batch* b, nflow_tuning* t) {
   if (!b || !t) return fail(NFLOW_EINVAL, "null argument");
   *t = b->b.tuning;
   return 0;
+}
+
+extern "C" int nflow_batch_step2_height(nflow_batch* b) {
+  if (!b) return fail(NFLOW_EINVAL, "null batch");
+  return b->b.step2_height();
 }
 
 extern "C" int nflow_batch_run(nflow_batch* b, int nsteps) {

@@ -105,6 +105,7 @@ SIGNATURES = {
     "nflow_batch_autotune": (_I, [_VP, _I, ctypes.POINTER(Tuning)]),
     "nflow_batch_set_tuning": (_I, [_VP, ctypes.POINTER(Tuning)]),
     "nflow_batch_get_tuning": (_I, [_VP, ctypes.POINTER(Tuning)]),
+    "nflow_batch_step2_height": (_I, [_VP]),
     "nflow_batch_run": (_I, [_VP, _I]),
     "nflow_batch_synchronize": (_I, [_VP]),
     "nflow_batch_read_arF": (_I, [_VP, _I, _FP]),
@@ -359,7 +360,7 @@ class Batch:
         _check(lib().nflow_batch_autotune(self.h, steps_per_trial, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, block=256, xcd_order=0, persistent=0, graph_steps=10, temporal=0):
+    def set_tuning(self, block=256, xcd_order=0, persistent=0, graph_steps=10, temporal=-1):
         t = Tuning(block, xcd_order, persistent, graph_steps, temporal, 0.0)
         _check(lib().nflow_batch_set_tuning(self.h, ctypes.byref(t)))
 
@@ -367,6 +368,10 @@ class Batch:
         t = Tuning()
         _check(lib().nflow_batch_get_tuning(self.h, ctypes.byref(t)))
         return t.as_dict()
+
+    def step2_height(self):
+        """y-chunk height of the two-step launches its runs use (0: one step per launch)."""
+        return _check(lib().nflow_batch_step2_height(self.h))
 
     def run(self, n):
         _check(lib().nflow_batch_run(self.h, n))

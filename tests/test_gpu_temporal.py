@@ -81,3 +81,25 @@ def test_temporal_is_ignored_where_it_does_not_apply():
     a = run(ch, nflow.F32, 0, [9], 0)[0][0]
     b = run(ch, nflow.F32, 0, [9], 16)[0][0]
     assert np.array_equal(a, b)
+
+
+def test_automatic_chunk_height():
+    """temporal = -1 (the default): two steps per launch where nx fills whole
+    256-wide tiles and a chunk height of 16-64 planes fills whole rounds of
+    one workgroup per CU (256 CUs)."""
+    def height(nx, ny, nz, storage=nflow.F32, casetype=nflow.CAVITY, temporal=-1):
+        b = nflow.Batch([nflow.defaults(**cfg(nx=nx, ny=ny, nz=nz, casetype=casetype), storage=storage)])
+        b.set_tuning(512, 0, 0, 0, temporal)
+        h = b.step2_height()
+        b.close()
+        return h
+    assert nflow.device_info()[1] == 256
+    assert height(256, 256, 256) == 32  # 32 tiles x 8 chunks
+    assert height(256, 128, 256) == 16
+    assert height(512, 256, 512) == 64  # 128 tiles x 4 chunks
+    assert height(128, 128, 128) == 0  # idle lanes
+    assert height(512, 64, 64) == 0  # no chunking fills whole rounds
+    assert height(256, 256, 256, storage=nflow.F16) == 0
+    assert height(256, 16, 256, casetype=nflow.CHANNEL) == 0
+    assert height(256, 256, 256, temporal=24) == 24
+    assert height(256, 256, 256, temporal=0) == 0
