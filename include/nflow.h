@@ -220,6 +220,9 @@ int nflow_batch_initialize(nflow_batch* b);
 int nflow_batch_autotune(nflow_batch* b, int steps_per_trial, nflow_tuning* chosen);
 int nflow_batch_set_tuning(nflow_batch* b, const nflow_tuning* t);
 int nflow_batch_get_tuning(nflow_batch* b, nflow_tuning* t);
+/* launch shape of a single-lattice solver (it is a batch of one; no
+ * reference counterpart): the same fields and rules as for a batch */
+int nflow_set_tuning(nflow_solver* s, const nflow_tuning* t);
 /* chunk height the batch's runs use for two steps per launch under its tuning
  * (temporal = -1 resolved), 0 when every launch is a single step */
 int nflow_batch_step2_height(nflow_batch* b);

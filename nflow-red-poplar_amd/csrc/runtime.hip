@@ -1021,6 +1021,11 @@ extern "C" int nflow_batch_get_tuning(nflow_batch* b, nflow_tuning* t) {
   return 0;
 }
 
+extern "C" int nflow_set_tuning(nflow_solver* s, const nflow_tuning* t) {
+  if (!s || !t) return fail(NFLOW_EINVAL, "null argument");
+  return s->b.set_tuning(*t);
+}
+
 extern "C" int nflow_batch_step2_height(nflow_batch* b) {
   if (!b) return fail(NFLOW_EINVAL, "null batch");
   return b->b.step2_height();

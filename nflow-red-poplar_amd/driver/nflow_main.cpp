@@ -36,7 +36,10 @@ static void usage() {
       "  --vtu FILE                     write the final fields as a VTK unstructured grid\n"
       "  --restart FILE                 continue from a checkpoint file (same extents)\n"
       "  --checkpoint FILE              save a checkpoint file at the end\n"
-      "  --no-print                     skip the arF slice print\n");
+      "  --no-print                     skip the arF slice print\n"
+      "  --block B                      threads per workgroup of the one-step launches (64..512)\n"
+      "  --temporal H                   two steps per launch in y chunks of H planes (FP32 cavity);\n"
+      "                                 0 = one step per launch, -1 = the library's choice (default)\n");
 }
 
 static int check(int rc, const char* what) {
@@ -50,7 +53,7 @@ static int check(int rc, const char* what) {
 int main(int argc, char** argv) {
   nflow_input in;
   nflow_input_defaults(&in);
-  int xd = 3, yd = 3, zd = 3, residual_every = 0;
+  int xd = 3, yd = 3, zd = 3, residual_every = 0, block = 0, temporal = -1;
   bool csv = false, print = true, tau_set = false;
   std::string vtu, restart, checkpoint;
   double re = 1000.0;
@@ -95,6 +98,8 @@ int main(int argc, char** argv) {
     else if (a == "--restart") restart = next();
     else if (a == "--checkpoint") checkpoint = next();
     else if (a == "--no-print") print = false;
+    else if (a == "--block") block = std::atoi(next());
+    else if (a == "--temporal") temporal = std::atoi(next());
     else { usage(); return a == "--help" ? 0 : 2; }
   }
   if (in.h != in.nx) in.h = in.nx;
@@ -111,6 +116,10 @@ int main(int argc, char** argv) {
 
   nflow_solver* s = nullptr;
   check(nflow_create(&in, ntile, sl, &s), "nflow_create");
+  if (block > 0 || temporal != -1) {
+    nflow_tuning t = {block > 0 ? block : (in.storage == NFLOW_STORAGE_F32 ? 256 : 512), 0, 0, 10, temporal, 0.0};
+    check(nflow_set_tuning(s, &t), "nflow_set_tuning");
+  }
   check(nflow_initialize(s), "nflow_initialize");
   if (!restart.empty()) {
     check(nflow_load_checkpoint(s, restart.c_str()), "nflow_load_checkpoint");

@@ -106,6 +106,7 @@ SIGNATURES = {
     "nflow_batch_set_tuning": (_I, [_VP, ctypes.POINTER(Tuning)]),
     "nflow_batch_get_tuning": (_I, [_VP, ctypes.POINTER(Tuning)]),
     "nflow_batch_step2_height": (_I, [_VP]),
+    "nflow_set_tuning": (_I, [_VP, ctypes.POINTER(Tuning)]),
     "nflow_batch_run": (_I, [_VP, _I]),
     "nflow_batch_synchronize": (_I, [_VP]),
     "nflow_batch_read_arF": (_I, [_VP, _I, _FP]),
@@ -241,6 +242,10 @@ class Solver:
     @property
     def shape(self):
         return (self.inp.ny, self.inp.nz, self.inp.nx)
+
+    def set_tuning(self, block=256, xcd_order=0, persistent=0, graph_steps=10, temporal=-1):
+        t = Tuning(block, xcd_order, persistent, graph_steps, temporal, 0.0)
+        _check(lib().nflow_set_tuning(self.h, ctypes.byref(t)))
 
     def initialize(self):
         _check(lib().nflow_initialize(self.h))

@@ -89,3 +89,17 @@ def test_pipe_case_writes_solid_flags(tmp_path):
     assert np.array_equal(arrays["Struct"].astype(bool), solid.ravel())
     assert not arrays["Density"][solid.ravel()].any()
     assert np.isfinite(arrays["Density"]).all()
+
+
+def test_tuning_options_keep_the_results(tmp_path):
+    """--block and --temporal change only the launches: the printed slice and
+    the result file are the same bit for bit (a 256-wide cavity, where two
+    steps per launch apply)."""
+    base = ["--nx", "256", "--ny", "20", "--nz", "16", "--les", "0.17", "--tau", "0.55", "--steps", "9", "--csv"]
+    outs = []
+    for opts in (["--temporal", "0"], ["--temporal", "4", "--block", "128"], ["--temporal", "1"]):
+        d = tmp_path / opts[1]
+        d.mkdir()
+        outs.append((printed_slice(run(base + opts, d)), (d / "result.csv").read_text()))
+    for got in outs[1:]:
+        assert np.array_equal(got[0], outs[0][0]) and got[1] == outs[0][1]
