@@ -67,6 +67,25 @@ def make_input(nflow, w):
     return p
 
 
+def timed_segments(k):
+    """Split k timed steps into 4 (k even) or 5 (k odd) runs of odd length:
+    with two steps per launch a run ends with one single step, and an
+    even-length run needs two."""
+    if k <= 5:
+        return [1] * k
+    nseg = 4 if k % 2 == 0 else 5
+    segs = [k // nseg + (1 if i < k % nseg else 0) for i in range(nseg)]
+    even = [i for i, v in enumerate(segs) if v % 2 == 0]  # an even count, since nseg = k (mod 2)
+    for a, b in zip(even[0::2], even[1::2]):
+        if segs[b] > 1:
+            segs[a] += 1
+            segs[b] -= 1
+        else:
+            segs[a] -= 1
+            segs[b] += 1
+    return [v for v in segs if v > 0]
+
+
 def traffic_from_profiles(name, kernel="k_step"):
     """HBM bytes per launch of the kernel measured by rocprofv3 --pmc, if recorded."""
     path = os.path.join(ROOT, "profiles", f"traffic_{name}{'_step2' if kernel == 'k_step2' else ''}.json")
@@ -514,9 +533,9 @@ def main():
     b.initialize()
     b.run(args.warmup)
     b.synchronize()
-    # K timed steps in five segments, each timed with HIP events on the
-    # solver's stream; the kernel time reported is the median segment's
-    segs = [args.steps // 5 + (1 if i < args.steps % 5 else 0) for i in range(5)]
+    # K timed steps in segments, each timed with HIP events on the solver's
+    # stream; the kernel time reported is the median segment's
+    segs = timed_segments(args.steps)
     seg_ms = []
     t0 = time.perf_counter()
     for k in segs:
@@ -578,7 +597,7 @@ def main():
         "data": "synthetic (lattice initialised at rest, rho = 1)",
         "config": {"workload": w["desc"], "nx": w["nx"], "ny": w["ny"], "nz": w["nz"], "nodes": n,
                    "tuning": dict(b.get_tuning(), ms_per_step=tuned_ms, autotuned=not args.no_autotune, step2_height=h2,
-                                  graph_replay=h2 == 0 and args.graph_steps > 0 and args.steps // 5 >= args.graph_steps),
+                                  graph_replay=h2 == 0 and args.graph_steps > 0 and min(segs, default=0) >= args.graph_steps),
                    "device": name, "cus": cus},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(args.workload, kn),

@@ -34,6 +34,8 @@ CASES = [
     cfg(nx=3, ny=2, nz=5, enable_les=1, smagorinsky=0.17, tau=0.7, init_lid_velocity=1),
     # wider than one x tile: halo columns between tiles, a partial last tile
     cfg(nx=300, ny=11, nz=20, enable_les=1, smagorinsky=0.17, tau=0.55, init_lid_velocity=1),
+    # two full x tiles, a partial z tile
+    cfg(nx=512, ny=7, nz=21, enable_les=1, smagorinsky=0.17, tau=0.58, init_lid_velocity=1),
 ]
 
 
