@@ -204,8 +204,8 @@ typedef struct nflow_tuning {
   int xcd_order;   /* 1: each XCD receives a contiguous range of work items */
   int persistent;  /* 0: one workgroup per work item; k > 0: k workgroups per CU, grid-stride */
   int graph_steps; /* steps per captured graph (even), 0 = no graph */
-  /* Two steps per launch for a batch of one FP32 cavity lattice without solid
-   * nodes (other batches ignore this): x-z tiles of 256 x 8 nodes swept along
+  /* Two steps per launch for a batch of one FP32 cavity or channel lattice
+   * without solid nodes (other batches ignore this): x-z tiles of 256 x 8 nodes swept along
    * y in chunks of H planes, the last launch of each run a single step, with
    * results bit for bit those of one step per launch.  H > 0: that chunk
    * height; -1 (the default): chosen by the library, on when nx is a multiple

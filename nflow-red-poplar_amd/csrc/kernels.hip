@@ -570,13 +570,21 @@ __device__ inline void collide_mrt(float f[Q], float rho, float ir, float jx, fl
 }
 
 // Macroscopic -> side planes -> LES -> collision, in place.
-// SIDES = false: for cavity lattices only (no side records to write).
+// SIDES = false: no side records written (cavity lattices, or a caller that
+// keeps them itself).  mac: if given, receives rho and u = j / rho as the side
+// records hold them.
 template <bool SIDES = true>
 __device__ __forceinline__ void relax(const LatticeDev& L, float f[Q], float* __restrict__ side_out, int x, int y,
-                                      int z) {
+                                      int z, float* mac = nullptr) {
   float rho, jx, jy, jz;
   moments(f, rho, jx, jy, jz);
   const float ir = hw_rcp(rho);
+  if (mac) {
+    mac[0] = rho;
+    mac[1] = jx * ir;
+    mac[2] = jy * ir;
+    mac[3] = jz * ir;
+  }
   if (SIDES && L.casetype != CASE_CAVITY) {  // channel and pipe: inlet / outlet side records
     const gptr<float> rec = (gptr<float>)side_out + (size_t)(y * L.nz + z) * SIDE_REC;
     if (x == 1) rec[SIDE_RHO] = rho;
@@ -702,7 +710,7 @@ __global__ void __launch_bounds__(256) k_advance(const LatticeDev* __restrict__ 
 #endif
 
 // ---------------------------------------------------------------------------
-// Two time steps per launch (a cavity lattice without solid nodes).
+// Two time steps per launch (a cavity or channel lattice without solid nodes).
 //
 // A workgroup owns a TX x TZ tile of the x-z plane and a range [ya, yb) of y
 // planes, and sweeps it in y.  TX spans whole x rows of the benchmark sizes:
@@ -756,7 +764,13 @@ struct Geo {
 };
 }  // namespace step2
 
-template <class St, class G>
+// CH: a channel lattice (velocity inlet at x = 0, pressure outlet at x =
+// nx - 1).  The second step's inlet and outlet rules read the first step's
+// rho at x = 1 and u at x = nx - 1 and nx - 2 on the same plane; the first
+// step leaves them in LDS (two plane slots), where k_step would have written
+// the side records, and the second step writes the side records the next
+// launch reads.
+template <class St, class G, bool CH>
 __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H, int ntx, int ntz) {
   static_assert(!St::shifted, "two steps per launch are built for FP32 storage");
   using namespace step2;
@@ -765,6 +779,9 @@ __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H
   // crossing populations of the own rows (with the x - 1 and x + TX columns)
   // and of the halo rows z0 - 1 (e_z = +1 ones) and z0 + TZ (e_z = -1 ones)
   __shared__ T cross[G::LN];
+  // channel: the first step's rho(x = 1), u(nx - 1), u(nx - 2) of each tile
+  // row, for planes of even and odd y
+  __shared__ float sidein[CH ? 2 : 1][TZ][8];
   int bt = blockIdx.x;
   // workgroups are dispatched to the 8 XCDs round-robin: give each XCD a
   // contiguous range of tiles, so that neighbours share their halo in L2
@@ -793,9 +810,21 @@ __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H
   // next step reads them back
   auto step1 = [&](int x, int yi, int z, T e[Q], float g[Q]) {
     const uint32_t n = node(x, yi, z);
-    pull<St, false>(L, in, n, n, x, yi, z, g);
-    boundary(L, SideIn{{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}}, x, yi, z, g);
-    relax<false>(L, g, nullptr, x, yi, z);
+    if constexpr (CH) {
+      gather<St, false>(L, in, L.side[p], n, n, x, yi, z, g);
+      float mac[4];
+      relax<false>(L, g, nullptr, x, yi, z, mac);
+      if (z >= z0 && z < z0 + TZ) {  // rows of the tile: what its second step reads
+        float* r = sidein[yi & 1][z - z0];
+        if (x == 1) r[0] = mac[0];
+        if (x == L.nx - 1) { r[1] = mac[1]; r[2] = mac[2]; r[3] = mac[3]; }
+        if (x == L.nx - 2) { r[4] = mac[1]; r[5] = mac[2]; r[6] = mac[3]; }
+      }
+    } else {
+      pull<St, false>(L, in, n, n, x, yi, z, g);
+      boundary(L, SideIn{{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}}, x, yi, z, g);
+      relax<false>(L, g, nullptr, x, yi, z);
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       e[q] = St::enc(g[q] - shift_of<St>(q));
@@ -876,8 +905,25 @@ __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H
         if (CIX[q] >= 0 && EY[q] < 0) fp[j][q] = gone(q, j, yf) ? fp[j][q] : pulled(q, j);
       if (xin && oz < L.nz && yf >= ya && yf < yb) {
         const uint32_t n = node(ox, yf, oz);
-        boundary(L, SideIn{{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}}, ox, yf, oz, fp[j]);
-        relax_and_store<St, false, false>(L, fp[j], out, nullptr, n, ox, yf, oz, 0u);
+        SideIn si = {{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}};
+        if constexpr (CH) {  // side_inputs(), from the first step's values
+          if ((ox == 0 || ox == L.nx - 1) && !(yf == 0 || yf == L.ny - 1 || oz == 0 || oz == L.nz - 1)) {
+            const float* r = sidein[yf & 1][zr0 + j];
+            if (ox == 0) {
+              si.v[0] = r[0];
+              si.v[1] = ((gptr<const float>)L.side[p])[(size_t)(yf * L.nz + oz) * SIDE_REC + SIDE_UIN];
+            } else {
+              si.v[0] = r[1];
+              si.v[1] = r[2];
+              si.v[2] = r[3];
+              si.v[3] = r[4];
+              si.v[4] = r[5];
+              si.v[5] = r[6];
+            }
+          }
+        }
+        boundary(L, si, ox, yf, oz, fp[j]);
+        relax_and_store<St, false, CH>(L, fp[j], out, L.side[p ^ 1], n, ox, yf, oz, 0u);
       }
     }
 #pragma unroll
@@ -1321,8 +1367,12 @@ hipError_t launch_step2(int storage, hipStream_t st, const LatticeDev& L, int pa
   using G = step2::Geo<STEP2_TX, STEP2_TZ, STEP2_NT>;
   if (H < 1 || storage != ST_F32) return hipErrorInvalidValue;
   const int ntx = (L.nx + G::TX - 1) / G::TX, ntz = (L.nz + G::TZ - 1) / G::TZ, nch = (L.ny + H - 1) / H;
-  hipLaunchKernelGGL((k_step2<StF32, G>), dim3((unsigned)(ntx * ntz * nch)), dim3(G::NT), 0, st, L, parity, H, ntx,
-                     ntz);
+  if (L.casetype == CASE_CHANNEL)
+    hipLaunchKernelGGL((k_step2<StF32, G, true>), dim3((unsigned)(ntx * ntz * nch)), dim3(G::NT), 0, st, L, parity, H,
+                       ntx, ntz);
+  else
+    hipLaunchKernelGGL((k_step2<StF32, G, false>), dim3((unsigned)(ntx * ntz * nch)), dim3(G::NT), 0, st, L, parity,
+                       H, ntx, ntz);
   return hipGetLastError();
 }
 

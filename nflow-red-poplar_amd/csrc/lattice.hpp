@@ -96,9 +96,10 @@ struct alignas(16) WorkItem {
 // nlat lattices.
 hipError_t launch_step(int storage, bool masked, bool sr, int block, int grid, hipStream_t st,
                        const LatticeDev* d_lats, int nlat, const WorkItem* d_items, int nitems, int parity);
-// Two steps in one launch (k_step2) for an FP32 cavity lattice without solid
-// nodes: reads buf[parity ^ parity0], writes the state two steps later to the
-// other buffer; y planes are swept in chunks of H.
+// Two steps in one launch (k_step2) for an FP32 cavity or channel lattice
+// without solid nodes: reads buf[parity ^ parity0] (and the side records of
+// that parity), writes the state two steps later to the other buffer (and
+// records); y planes are swept in chunks of H.
 hipError_t launch_step2(int storage, hipStream_t st, const LatticeDev& L, int parity, int H);
 // step0: the step count the initialised (or imported) state stands at; the
 // stochastic rounding of its look-ahead collision uses it.

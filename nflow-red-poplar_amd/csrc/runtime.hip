@@ -384,7 +384,7 @@ int Batch::launch_steps(int nsteps) {
   int rc = upload_tables();
   if (rc) return rc;
   int left = nsteps;
-  // two steps per launch: one FP32 cavity lattice without solid nodes; the
+  // two steps per launch: one FP32 cavity or channel lattice without solid nodes; the
   // last launch stays a single step so that the read-back finds the state it
   // streams (the intermediate state of a pair never reaches memory)
   const int h2 = step2_height();
@@ -589,7 +589,7 @@ int Batch::step2_height(const nflow_tuning* t) const {
   const nflow_tuning& tu = t ? *t : tuning;
   if (tu.temporal == 0 || storage != NFLOW_STORAGE_F32 || lats.size() != 1) return 0;
   const HostLattice& h = lats[0];
-  if (h.in.casetype != NFLOW_CAVITY || h.dev.mask) return 0;
+  if (h.in.casetype == NFLOW_PIPE || h.dev.mask) return 0;
   if (tu.temporal > 0) return tu.temporal;
   // automatic: whole tile rows in x (no idle lanes), and the fewest y chunks
   // of 16 to 64 planes whose tiles fill whole rounds of one workgroup per CU

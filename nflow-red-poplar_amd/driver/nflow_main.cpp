@@ -38,7 +38,7 @@ static void usage() {
       "  --checkpoint FILE              save a checkpoint file at the end\n"
       "  --no-print                     skip the arF slice print\n"
       "  --block B                      threads per workgroup of the one-step launches (64..512)\n"
-      "  --temporal H                   two steps per launch in y chunks of H planes (FP32 cavity);\n"
+      "  --temporal H                   two steps per launch in y chunks of H planes (FP32 cavity, channel);\n"
       "                                 0 = one step per launch, -1 = the library's choice (default)\n");
 }
 

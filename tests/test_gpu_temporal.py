@@ -1,9 +1,9 @@
-"""Two time steps per launch (nflow_tuning.temporal, k_step2; FP32 cavity
-lattices) against one step per launch: the same state bit for bit, for both
-collision models, partial tiles and y chunks, several x tiles, tiny boxes,
-the benchmark size, split runs, and a read-back after every run (the last
-launch of a run is a single step, so the export finds the state it
-streams)."""
+"""Two time steps per launch (nflow_tuning.temporal, k_step2; FP32 cavity and
+channel lattices) against one step per launch: the same state bit for bit,
+for both collision models, partial tiles and y chunks, several x tiles, the
+channel's inlet and outlet in one or two tiles, tiny boxes, the benchmark
+size, split runs, and a read-back after every run (the last launch of a run
+is a single step, so the export finds the state it streams)."""
 import numpy as np
 import pytest
 
@@ -36,6 +36,13 @@ CASES = [
     cfg(nx=300, ny=11, nz=20, enable_les=1, smagorinsky=0.17, tau=0.55, init_lid_velocity=1),
     # two full x tiles, a partial z tile
     cfg(nx=512, ny=7, nz=21, enable_les=1, smagorinsky=0.17, tau=0.58, init_lid_velocity=1),
+    # channels: inlet and outlet in one tile; the outlet in a second tile; the
+    # outlet alone in the second tile (u(nx - 2) from the halo column)
+    cfg(nx=40, ny=12, nz=10, casetype=nflow.CHANNEL, enable_les=1, smagorinsky=0.17, tau=0.56, inlet_peak=0.05,
+        perturb=0.01, seed=3),
+    cfg(nx=300, ny=9, nz=18, casetype=nflow.CHANNEL, enable_les=1, smagorinsky=0.17, tau=0.56, inlet_peak=0.05,
+        perturb=0.01, seed=5),
+    cfg(nx=257, ny=10, nz=11, casetype=nflow.CHANNEL, enable_mrt=0, tau=0.6, inlet_peak=0.05, perturb=0.01, seed=7),
 ]
 
 
@@ -77,12 +84,23 @@ def test_temporal_is_ignored_for_16_bit_storage(storage, sr):
 
 
 def test_temporal_is_ignored_where_it_does_not_apply():
-    # a channel lattice keeps one step per launch
-    ch = cfg(nx=40, ny=12, nz=10, casetype=nflow.CHANNEL, enable_les=1, smagorinsky=0.17, tau=0.56,
-             inlet_peak=0.05, perturb=0.01, seed=3)
-    a = run(ch, nflow.F32, 0, [9], 0)[0][0]
-    b = run(ch, nflow.F32, 0, [9], 16)[0][0]
+    # a pipe (solid nodes) and a batch of two lattices keep one step per launch
+    pipe = cfg(nx=30, ny=20, nz=20, casetype=nflow.PIPE, pipe_radius=8, pipe_cy=10, pipe_cz=10, tau=0.6)
+    a = run(pipe, nflow.F32, 0, [9], 0)[0][0]
+    b = run(pipe, nflow.F32, 0, [9], 16)[0][0]
     assert np.array_equal(a, b)
+    two = [nflow.defaults(**c, storage=nflow.F32) for c in CASES[:2]]
+    out = []
+    for h in (0, 16):
+        bt = nflow.Batch(two)
+        bt.set_tuning(256, 0, 0, 0, h)
+        assert bt.step2_height() == 0
+        bt.initialize()
+        bt.run(9)
+        out.append([bt.read_arF(i) for i in range(2)])
+        bt.close()
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
 
 
 def test_automatic_chunk_height():
@@ -102,6 +120,7 @@ def test_automatic_chunk_height():
     assert height(128, 128, 128) == 0  # idle lanes
     assert height(512, 64, 64) == 0  # no chunking fills whole rounds
     assert height(256, 256, 256, storage=nflow.F16) == 0
-    assert height(256, 16, 256, casetype=nflow.CHANNEL) == 0
+    assert height(512, 128, 128, casetype=nflow.CHANNEL) == 16  # 32 tiles x 8 chunks
+    assert height(512, 128, 128, casetype=nflow.PIPE) == 0
     assert height(256, 256, 256, temporal=24) == 24
     assert height(256, 256, 256, temporal=0) == 0
