@@ -732,12 +732,14 @@ __global__ void __launch_bounds__(256) k_advance(const LatticeDev* __restrict__ 
 //      complete, stored to buf[p ^ 1].
 // The partial pulls of planes yi and yi + 1 stay in registers.
 //
-// Per node and step the arithmetic, the rounding of the stored value and its
-// seed are those of k_step, so two launches of k_step and one of k_step2 give
-// the same state bit for bit; the memory traffic of the pair is one read and
-// one write of the populations (plus the halo) instead of two of each.  The
-// intermediate state never reaches memory, so the launch before a read-back
-// must be a k_step (the export streams the state the last launch read).
+// Built for FP32 storage.  Per node and step the arithmetic and the stored
+// values are those of k_step (both compiled with -ffp-contract=on, so an FMA
+// never spans statements and the collision rounds the same in either kernel),
+// so two launches of k_step and one of k_step2 give the same state bit for
+// bit; the memory traffic of the pair is one read and one write of the
+// populations (plus the halo) instead of two of each.  The intermediate state
+// never reaches memory, so the launch before a read-back must be a k_step
+// (the export streams the state the last launch read).
 // ---------------------------------------------------------------------------
 namespace step2 {
 // Slot of crossing population q (e_x or e_z non-zero) in an LDS row: the 5
