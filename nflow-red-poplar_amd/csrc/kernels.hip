@@ -193,6 +193,17 @@ __device__ constexpr float shift_of(int q) {
   return St::shifted ? W[q] : 0.0f;
 }
 
+// Stored element of population q -> its value: f = v + w_q for the shifted
+// 16-bit formats; FP32 stores f itself (an added 0.0f would only turn -0 into
+// +0, at one VALU instruction per population).
+template <class St>
+__device__ __forceinline__ float load_f(typename St::T v, int q) {
+  if constexpr (St::shifted)
+    return St::dec(v) + shift_of<St>(q);
+  else
+    return St::dec(v);
+}
+
 // Seeds of the stochastic rounding.  Each collision draws one 32-bit hash per
 // node from (node id, collision count) and gives store q the seed
 // hash + q * 0x9E3779B9 (a Weyl sequence, whose top bits -- the ones the
@@ -278,7 +289,7 @@ __device__ inline void pull(const LatticeDev& L, const Acc& in, uint32_t n, uint
     }
     const uint32_t idx =
         out ? pidx<typename St::T>(OPP[q], nr, ps) : pidx<typename St::T>(q, (uint32_t)((int)nr - off), ps);
-    f[q] = St::dec(in.get(idx)) + shift_of<St>(q);
+    f[q] = load_f<St>(in.get(idx), q);
   }
 }
 
@@ -830,7 +841,7 @@ __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       e[q] = St::enc(g[q] - shift_of<St>(q));
-      g[q] = St::dec(e[q]) + shift_of<St>(q);
+      g[q] = load_f<St>(e[q], q);
     }
   };
 
@@ -894,10 +905,10 @@ __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H
     __syncthreads();
     // crossing population q pulled by own node j from plane yi
     auto pulled = [&](int q, int j) {
-      return St::dec(cross[G::lidx(zr0 + j - EZ[q], CIX[q], lx + 1 - EX[q])]) + shift_of<St>(q);
+      return load_f<St>(cross[G::lidx(zr0 + j - EZ[q], CIX[q], lx + 1 - EX[q])], q);
     };
     // own node j's crossing population q after the first step on plane yi
-    auto own = [&](int q, int j) { return St::dec(cross[G::lidx(zr0 + j, CIX[q], lx + 1)]) + shift_of<St>(q); };
+    auto own = [&](int q, int j) { return load_f<St>(cross[G::lidx(zr0 + j, CIX[q], lx + 1)], q); };
     const int yf = yi - 1;
 #pragma unroll
     for (int j = 0; j < K; ++j) {  // second step on plane yf
@@ -1058,7 +1069,7 @@ __global__ void __launch_bounds__(EXPORT_BS) k_export(LatticeDev L, int prev, in
     float f[Q];
     if (prev_streamed) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) f[q] = St::dec(pb.get(pidx<typename St::T>(q, n, ps))) + shift_of<St>(q);
+      for (int q = 0; q < Q; ++q) f[q] = load_f<St>(pb.get(pidx<typename St::T>(q, n, ps)), q);
     } else {
       gather<St, true>(L, pb, L.side[prev], n, n, x, y, z, f);
     }
@@ -1086,7 +1097,7 @@ __global__ void __launch_bounds__(EXPORT_BS) k_export(LatticeDev L, int prev, in
       if (!prev_streamed) {
         // the collision conserves mass, so sum_q f*_q(prev) is the previous step's density
 #pragma unroll
-        for (int q = 0; q < Q; ++q) ro += (double)(St::dec(pb.get(pidx<typename St::T>(q, n, ps))) + shift_of<St>(q));
+        for (int q = 0; q < Q; ++q) ro += (double)load_f<St>(pb.get(pidx<typename St::T>(q, n, ps)), q);
       }
       dm = rn;
       dx = (double)jx;
