@@ -578,13 +578,6 @@ int Batch::node_types(int idx, int* type, int* dirstate) {
   return 0;
 }
 
-// Times every candidate launch shape on this batch's own lattices and keeps
-// the fastest.  The clocks of an issue-bound kernel settle over the first
-// few hundred steps, which would favour whichever shapes run first, so the
-// device is warmed up with the first candidate and the candidates are timed
-// twice, in opposite orders, each keeping its better time.  The trial steps
-// advance the state, so every lattice is re-initialised afterwards (call
-// before importing a state).
 int Batch::step2_height(const nflow_tuning* t) const {
   const nflow_tuning& tu = t ? *t : tuning;
   if (tu.temporal == 0 || storage != NFLOW_STORAGE_F32 || lats.size() != 1) return 0;
@@ -593,8 +586,9 @@ int Batch::step2_height(const nflow_tuning* t) const {
   if (tu.temporal > 0) return tu.temporal;
   // automatic: whole tile rows in x (no idle lanes), and the fewest y chunks
   // of 16 to 64 planes whose tiles fill whole rounds of one workgroup per CU
-  // (measured at 256^3, 256 x 128 x 256 and 512 x 256 x 512: 1.31-1.43x the
-  // one-step rate; with idle lanes or a partial last round it can be slower)
+  // (cavities of 256^3, 256 x 128 x 256 and 512 x 256 x 512 measured at
+  // 1.31-1.43x the one-step rate, channels at 1.13-1.25x; with idle lanes or
+  // a partial last round it can be slower)
   if (h.in.nx % STEP2_TX != 0) return 0;
   const long cols = (long)(h.in.nx / STEP2_TX) * ((h.in.nz + STEP2_TZ - 1) / STEP2_TZ);
   for (int nch = 1; nch <= h.in.ny; ++nch) {
@@ -606,6 +600,13 @@ int Batch::step2_height(const nflow_tuning* t) const {
   return 0;
 }
 
+// Times every candidate launch shape on this batch's own lattices and keeps
+// the fastest.  The clocks of an issue-bound kernel settle over the first
+// few hundred steps, which would favour whichever shapes run first, so the
+// device is warmed up with the first candidate and the candidates are timed
+// twice, in opposite orders, each keeping its better time.  The trial steps
+// advance the state, so every lattice is re-initialised afterwards (call
+// before importing a state).
 int Batch::autotune(int trial_steps, nflow_tuning* out) {
   if (trial_steps < 2) trial_steps = 2;
   const nflow_tuning base = tuning;
