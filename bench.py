@@ -489,8 +489,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="fixed launch shape instead of the autotuner's")
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--temporal", type=int, default=None,
-                    help="FP32 cavity: two steps per launch, y swept in chunks of this many planes (0: one step per "
-                         "launch, -1: the library's choice; default: the autotuner's)"),
+                    help="FP32 cavity or channel: two steps per launch, y swept in chunks of this many planes (0: one "
+                         "step per launch, -1: the library's choice; default: the autotuner's)")
     ap.add_argument("--block", type=int, default=512, help="workgroup size with --no-autotune")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--requests", type=int, default=48, help="serve mode: length of the request stream")
