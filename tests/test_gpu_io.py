@@ -72,3 +72,35 @@ def test_vtu_of_device_fields(tmp_path):
     npts, ncell, a = parse_vtu(p)
     assert npts == 9 * 8 * 7 and np.array_equal(a["Density"], r.ravel())
     assert a["Moving Wall"].sum() == (9 - 2) * (7 - 2)
+
+
+@pytest.mark.parametrize("casetype", [nflow.CAVITY, nflow.CHANNEL])
+def test_checkpoint_restart_with_two_steps_per_launch(tmp_path, casetype):
+    """A run with two steps per launch saved after an odd and an even number of
+    steps, restarted, and continued equals one uninterrupted run with one step
+    per launch, bit for bit (the export after each run reads the state of its
+    last, single-step launch)."""
+    c = cfg(nx=256, ny=14, nz=11, enable_les=1, smagorinsky=0.17, tau=0.57, casetype=casetype,
+            inlet_peak=0.05, perturb=0.01, seed=11)
+    ref = nflow.Solver(nflow.defaults(**c, storage=nflow.F32))
+    ref.set_tuning(256, 0, 0, 10, 0)
+    ref.initialize()
+    ref.run(40)
+    want = ref.read_arF()
+    ref.close()
+    for first in (17, 20):
+        s = nflow.Solver(nflow.defaults(**c, storage=nflow.F32))
+        s.set_tuning(256, 0, 0, 10, 4)
+        s.initialize()
+        s.run(first)
+        p = tmp_path / f"two_{first}.ckp"
+        s.save_checkpoint(str(p))
+        s.close()
+        t = nflow.Solver(nflow.defaults(**c, storage=nflow.F32))
+        t.set_tuning(256, 0, 0, 10, 4)
+        t.initialize()
+        t.load_checkpoint(str(p))
+        t.run(40 - first)
+        assert t.steps_done() == 40
+        assert np.array_equal(t.read_arF(), want), first
+        t.close()
