@@ -68,11 +68,15 @@ def make_input(nflow, w):
 
 
 def timed_segments(k):
-    """Split k timed steps into 4 (k even) or 5 (k odd) runs of odd length:
-    with two steps per launch a run ends with one single step, and an
-    even-length run needs two."""
+    """Split k timed steps into runs of odd length: with two steps per launch
+    a run ends with one single step, and an even-length run needs two.  Four
+    (k even) or five (k odd) runs from 100 steps on, for a median; below
+    that as few runs as the parity allows (one or two), so that single steps
+    stay a small share."""
     if k <= 5:
         return [1] * k
+    if k < 100:
+        return [k] if k % 2 else [k // 2 - (k // 2 + 1) % 2, k // 2 + (k // 2 + 1) % 2]
     nseg = 4 if k % 2 == 0 else 5
     segs = [k // nseg + (1 if i < k % nseg else 0) for i in range(nseg)]
     even = [i for i, v in enumerate(segs) if v % 2 == 0]  # an even count, since nseg = k (mod 2)

@@ -22,7 +22,8 @@ def test_timed_segments_are_odd_and_sum_to_k():
         assert all(v > 0 and v % 2 == 1 for v in segs), (k, segs)
         assert len(segs) <= 5, (k, segs)
     assert b.timed_segments(2000) == [501, 499, 501, 499]
-    assert b.timed_segments(20) == [5, 5, 5, 5]
+    assert b.timed_segments(20) == [9, 11]
+    assert b.timed_segments(21) == [21]
     # near-equal runs: the K = 2000 default's median segment is a full run
     for k in (200, 2000, 2001, 20000):
         segs = b.timed_segments(k)
