@@ -802,7 +802,7 @@ __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H
   static_assert(!St::shifted, "two steps per launch are built for FP32 storage");
   using namespace step2;
   using T = typename St::T;
-  constexpr int TX = G::TX, TZ = G::TZ, NT = G::NT, WX = G::WX, K = G::K;
+  constexpr int TX = G::TX, TZ = G::TZ, NT = G::NT, K = G::K;
   // crossing populations of the own rows (with the x - 1 and x + TX columns)
   // and of the halo rows z0 - 1 (e_z = +1 ones) and z0 + TZ (e_z = -1 ones)
   __shared__ T cross[G::LN];
