@@ -76,6 +76,29 @@ def test_two_steps_per_launch_at_256_cubed():
     assert d[0] == d2[0] and np.array_equal(d[1], d2[1]) and d[2] == d2[2]
 
 
+def test_two_steps_per_launch_on_the_512x128x128_channel():
+    """The channel of the precision runs at its automatic chunk height (16):
+    two x tiles, inlet and outlet in different tiles, 8 y chunks."""
+    c = cfg(nx=512, ny=128, nz=128, casetype=nflow.CHANNEL, enable_les=1, smagorinsky=0.17, tau=0.56,
+            inlet_peak=0.05, perturb=0.01, seed=11)
+    out = []
+    for h in (0, -1):
+        b = nflow.Batch([nflow.defaults(**c, storage=nflow.F32)])
+        b.set_tuning(512, 0, 0, 0, h)
+        assert b.step2_height() == (0 if h == 0 else 16)
+        b.initialize()
+        got = []
+        for n in (9, 6):
+            b.run(n)
+            got.append((b.read_macro(0), b.diagnostics(0)))
+        b.close()
+        out.append(got)
+    for (m, d), (m2, d2) in zip(*out):
+        for x, y in zip(m, m2):
+            assert np.array_equal(x, y)
+        assert d[0] == d2[0] and np.array_equal(d[1], d2[1]) and d[2] == d2[2]
+
+
 @pytest.mark.parametrize("storage,sr", [(nflow.F16, 1), (nflow.BF16, 0)])
 def test_temporal_is_ignored_for_16_bit_storage(storage, sr):
     a = run(CASES[0], storage, sr, [9], 0)[0][0]
