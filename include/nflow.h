@@ -67,8 +67,8 @@ extern "C" {
 typedef struct nflow_input {
   int h;                 /* Inputval.h: reference length (main.cpp:59) */
   int nx, ny, nz;        /* interior nodes per axis */
-  int maxitr;            /* steps the driver runs after initialisation */
-  int nwrite;            /* output period of the driver (kernel.cu:205) */
+  int maxitr;            /* steps nflow_main runs after initialisation (main.cpp:291) */
+  int nwrite;            /* output period of nflow_main (kernel.cu:205) */
   int casetype;          /* NFLOW_PIPE (0), NFLOW_CAVITY (1) or NFLOW_CHANNEL (2) */
   int enable_mrt;        /* 1: MRT collision, 0: SRT/BGK */
   int enable_les;        /* Smagorinsky eddy viscosity */

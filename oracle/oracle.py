@@ -1,8 +1,9 @@
 """ctypes access to the FP64 oracle (liboracle.so) and to oracle/_ref/libref.so.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product library never
-imports this module.
+cpu_baseline leg of bench.py, as the reference that GPU results are compared
+with (and, in bench.py, timed beside).  The product library never imports this
+module.
 """
 import ctypes
 import os
