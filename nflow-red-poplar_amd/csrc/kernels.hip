@@ -78,6 +78,19 @@ __device__ __forceinline__ uint32_t pidx(int q, uint32_t n, uint32_t ps) {
 #endif
 }
 
+// Element offset of node bn's block (bn a multiple of the block length B):
+// pidx(q, n) = window(bn) + pidx(q, n - bn), so a kernel that views a
+// buffer from that element on indexes it with 32-bit offsets whatever the
+// lattice size (a lattice may hold more than 2^32 / 19 nodes).
+template <class T>
+__device__ __forceinline__ size_t window(uint32_t bn) {
+#if NFLOW_AOSOA
+  return (size_t)bn * Q;
+#else
+  return bn;
+#endif
+}
+
 // The population buffers live in device global memory; saying so (address
 // space 1) lets the compiler emit global_load/global_store instead of FLAT
 // accesses, which it must otherwise use for pointers read out of a struct.
@@ -286,7 +299,7 @@ __device__ inline void pull(const LatticeDev& L, const Acc& in, uint32_t n, uint
     // registers)
     const int off = EX[q] + EZ[q] * L.nx + EY[q] * L.nxz;
     if (MASK && q != 0) {
-      if (mk && !out) out = mk[(int)n - off] != 0;
+      if (mk && !out) out = mk[(long long)n - off] != 0;
     }
     const uint32_t idx =
         out ? pidx<typename St::T>(OPP[q], nr, ps) : pidx<typename St::T>(q, (uint32_t)((int)nr - off), ps);
@@ -702,8 +715,12 @@ __global__ void __launch_bounds__(BS) NFLOW_STEP_ATTR k_step(const LatticeDev* _
       step_node<St, MASK, SR>(L, wi, p, bn, BPops<T>(static_cast<T*>(L.buf[p]) + skip, bytes),
                           BPops<T>(static_cast<T*>(L.buf[p ^ 1]) + skip, bytes), sr_key);
     } else {
-      step_node<St, MASK, SR>(L, wi, p, 0u, GPops<T>{static_cast<T*>(L.buf[p])}, GPops<T>{static_cast<T*>(L.buf[p ^ 1])},
-                              sr_key);
+      // the same window, from a per-item base pointer
+      constexpr uint32_t B = 256 / sizeof(T);
+      const uint32_t reach = (uint32_t)L.nxz + (uint32_t)L.nx;
+      const uint32_t bn = (wi.start > reach ? wi.start - reach : 0u) & ~(B - 1);
+      step_node<St, MASK, SR>(L, wi, p, bn, GPops<T>{static_cast<T*>(L.buf[p]) + window<T>(bn)},
+                              GPops<T>{static_cast<T*>(L.buf[p ^ 1]) + window<T>(bn)}, sr_key);
     }
   }
 }
@@ -995,7 +1012,9 @@ __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H
 // and the first step launch reads step0 + 1 from ctr[cur].
 template <class St>
 __global__ void __launch_bounds__(256) k_init(LatticeDev L, int cur, const float* __restrict__ src, uint32_t step0) {
-  const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+  using T = typename St::T;
+  const uint32_t bn = blockIdx.x * 256u;  // the block's first node: buffers viewed from there
+  const uint32_t n = bn + threadIdx.x, nr = threadIdx.x;
   if (n >= L.n) return;
   if (n == 0) L.ctr[cur] = step0 + 1u;
   int x, y, z;
@@ -1005,11 +1024,11 @@ __global__ void __launch_bounds__(256) k_init(LatticeDev L, int cur, const float
     // kernel.cu:294, leaving the memset zeros) and rho = u = 0 in both side
     // buffers, never touched again
     const uint32_t ps = (uint32_t)L.pstride;
-    const GPops<typename St::T> b0{static_cast<typename St::T*>(L.buf[0])}, b1{static_cast<typename St::T*>(L.buf[1])};
+    const GPops<T> b0{static_cast<T*>(L.buf[0]) + window<T>(bn)}, b1{static_cast<T*>(L.buf[1]) + window<T>(bn)};
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      b0.put(pidx<typename St::T>(q, n, ps), St::enc(-shift_of<St>(q)));
-      b1.put(pidx<typename St::T>(q, n, ps), St::enc(-shift_of<St>(q)));
+      b0.put(pidx<T>(q, nr, ps), St::enc(-shift_of<St>(q)));
+      b1.put(pidx<T>(q, nr, ps), St::enc(-shift_of<St>(q)));
     }
     if (L.casetype != CASE_CAVITY && (x == 1 || x >= L.nx - 2)) {
       for (int b = 0; b < 2; ++b) {
@@ -1040,12 +1059,12 @@ __global__ void __launch_bounds__(256) k_init(LatticeDev L, int cur, const float
       f[q] = W[q] * L.refden * (1.0f + 3.0f * eu + 4.5f * eu * eu - 1.5f * usq);
     }
   }
-  const GPops<typename St::T> streamed{static_cast<typename St::T*>(L.buf[cur ^ 1])};
+  const GPops<T> streamed{static_cast<T*>(L.buf[cur ^ 1]) + window<T>(bn)};
   const uint32_t ps = (uint32_t)L.pstride;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) streamed.put(pidx<typename St::T>(q, n, ps), St::enc(f[q] - shift_of<St>(q)));
-  relax_and_store<St, true>(L, f, GPops<typename St::T>{static_cast<typename St::T*>(L.buf[cur])}, L.side[cur], n, x, y,
-                            z, mix32(n ^ mix32(step0)));
+  for (int q = 0; q < Q; ++q) streamed.put(pidx<T>(q, nr, ps), St::enc(f[q] - shift_of<St>(q)));
+  relax_and_store<St, true>(L, f, GPops<T>{static_cast<T*>(L.buf[cur]) + window<T>(bn)}, L.side[cur], nr, x, y, z,
+                            mix32(n ^ mix32(step0)));
 }
 
 // ---------------------------------------------------------------------------
@@ -1087,14 +1106,20 @@ __global__ void __launch_bounds__(EXPORT_BS) k_export(LatticeDev L, int prev, in
   } else if (n < L.n) {
     int x, y, z;
     coords(L, n, x, y, z);
-    const GPops<typename St::T> pb{static_cast<typename St::T*>(L.buf[prev])};
+    // buf[prev] viewed from the block of the farthest donor of the
+    // workgroup's first node (32-bit offsets from there)
+    using T = typename St::T;
+    constexpr uint32_t B = 256 / sizeof(T);
+    const uint32_t first = blockIdx.x * (uint32_t)EXPORT_BS, reach = (uint32_t)L.nxz + (uint32_t)L.nx;
+    const uint32_t bn = (first > reach ? first - reach : 0u) & ~(B - 1), nr = n - bn;
+    const GPops<T> pb{static_cast<T*>(L.buf[prev]) + window<T>(bn)};
     const uint32_t ps = (uint32_t)L.pstride;
     float f[Q];
     if (prev_streamed) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) f[q] = load_f<St>(pb.get(pidx<typename St::T>(q, n, ps)), q);
+      for (int q = 0; q < Q; ++q) f[q] = load_f<St>(pb.get(pidx<T>(q, nr, ps)), q);
     } else {
-      gather<St, true>(L, pb, L.side[prev], n, n, x, y, z, f);
+      gather<St, true>(L, pb, L.side[prev], n, nr, x, y, z, f);
     }
     float rho, jx, jy, jz;
     moments(f, rho, jx, jy, jz);
@@ -1120,7 +1145,7 @@ __global__ void __launch_bounds__(EXPORT_BS) k_export(LatticeDev L, int prev, in
       if (!prev_streamed) {
         // the collision conserves mass, so sum_q f*_q(prev) is the previous step's density
 #pragma unroll
-        for (int q = 0; q < Q; ++q) ro += (double)load_f<St>(pb.get(pidx<typename St::T>(q, n, ps)), q);
+        for (int q = 0; q < Q; ++q) ro += (double)load_f<St>(pb.get(pidx<T>(q, nr, ps)), q);
       }
       dm = rn;
       dx = (double)jx;
@@ -1179,9 +1204,10 @@ __global__ void __launch_bounds__(512) k_probe(const LatticeDev* __restrict__ la
     const WorkItem wi = items[it];
     if (threadIdx.x >= wi.count) continue;
     const LatticeDev& L = lats[wi.lat];
-    const uint32_t n = wi.start + threadIdx.x;
-    const GPops<T> b{static_cast<T*>(L.buf[parity ^ L.parity0])};
-    const float v = St::dec(b.get(pidx<T>(0, n, (uint32_t)L.pstride)));
+    constexpr uint32_t B = 256 / sizeof(T);
+    const uint32_t bn = wi.start & ~(B - 1);
+    const GPops<T> b{static_cast<T*>(L.buf[parity ^ L.parity0]) + window<T>(bn)};
+    const float v = St::dec(b.get(pidx<T>(0, wi.start + threadIdx.x - bn, (uint32_t)L.pstride)));
     if (!__builtin_isfinite(v)) L.ctr[CTR_DIVERGED] = 1u;
   }
 }
@@ -1401,7 +1427,8 @@ hipError_t launch_step(int storage, bool masked, bool sr, int block, int grid, h
 
 hipError_t launch_step2(int storage, hipStream_t st, const LatticeDev& L, int parity, int H) {
   using G = step2::Geo<STEP2_TX, STEP2_TZ, STEP2_NT>;
-  if (H < 1 || storage != ST_F32) return hipErrorInvalidValue;
+  // k_step2 indexes the whole buffers with 32-bit element indices
+  if (H < 1 || storage != ST_F32 || (unsigned long long)L.pstride * Q >= (1ull << 32)) return hipErrorInvalidValue;
   const int ntx = (L.nx + G::TX - 1) / G::TX, ntz = (L.nz + G::TZ - 1) / G::TZ, nch = (L.ny + H - 1) / H;
   if (L.casetype == CASE_CHANNEL)
     hipLaunchKernelGGL((k_step2<StF32, G, true>), dim3((unsigned)(ntx * ntz * nch)), dim3(G::NT), 0, st, L, parity, H,

@@ -26,6 +26,7 @@ namespace nflow {
 #endif
 
 static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+static constexpr unsigned long long NODE_LIMIT = (1ull << 31) - (1ull << 20);
 static size_t elem_size(int storage) { return storage == NFLOW_STORAGE_F32 ? 4 : 2; }
 
 int validate_input(const nflow_input& in, int storage) {
@@ -37,8 +38,9 @@ int validate_input(const nflow_input& in, int storage) {
     return fail(NFLOW_EINVAL, "pipe_radius must be positive (got %g)", in.pipe_radius);
   if (storage < NFLOW_STORAGE_F32 || storage > NFLOW_STORAGE_BF16) return fail(NFLOW_EINVAL, "bad storage");
   const unsigned long long n = (unsigned long long)in.nx * in.ny * in.nz;
-  const unsigned long long pstride = round_up(n, 128) + NFLOW_PLANE_PAD;
-  if (pstride * 19ull >= (1ull << 32)) return fail(NFLOW_EINVAL, "lattice of %llu nodes exceeds the 32-bit index range", n);
+  // node ids are 32-bit and divided exactly by FastDiv below 2^31; the
+  // populations are addressed through windows with 32-bit offsets
+  if (n >= NODE_LIMIT) return fail(NFLOW_EINVAL, "lattice of %llu nodes exceeds the %llu node ids the kernels address", n, NODE_LIMIT);
   // the step kernel addresses a window of two x-z planes plus one work item
   // with 32-bit byte offsets
   const unsigned long long plane = (unsigned long long)in.nx * in.nz + in.nx;
@@ -583,6 +585,8 @@ int Batch::step2_height(const nflow_tuning* t) const {
   if (tu.temporal == 0 || storage != NFLOW_STORAGE_F32 || lats.size() != 1) return 0;
   const HostLattice& h = lats[0];
   if (h.in.casetype == NFLOW_PIPE || h.dev.mask) return 0;
+  // k_step2 indexes the whole buffers with 32-bit element indices
+  if ((unsigned long long)h.dev.pstride * 19ull >= (1ull << 32)) return 0;
   if (tu.temporal > 0) return tu.temporal;
   // automatic: whole tile rows in x (no idle lanes), and the fewest y chunks
   // of 16 to 64 planes whose tiles fill whole rounds of one workgroup per CU

@@ -70,7 +70,8 @@ def gpu_present():
 
 def test_invalid_inputs_are_refused_before_the_device():
     for kw in (dict(tau=0.5), dict(nx=0), dict(casetype=7), dict(storage=9), dict(casetype=nflow.CHANNEL, nx=2),
-               dict(ref_den=0.0), dict(nx=1024, ny=1024, nz=1024), dict(nx=512, ny=863, nz=512)):
+               dict(ref_den=0.0), dict(nx=2048, ny=2048, nz=2048),
+               dict(nx=1024, ny=2047, nz=1024)):  # 2^31 - 2^20 nodes: the first count refused
         with pytest.raises(nflow.NflowError) as e:
             nflow.Solver(nflow.defaults(**{**dict(nx=8, ny=8, nz=8), **kw}))
         assert e.value.code == -1, kw

@@ -51,16 +51,18 @@ def test_cavity_256_is_deterministic(storage):
 
 
 @pytest.mark.parametrize("storage", [nflow.F32, nflow.F16])
-def test_largest_accepted_lattice_matches_a_short_box(storage):
-    """512 x 862 x 512 (226 M nodes) is the largest box of that x-z extent the
-    32-bit element indexing accepts (19 round_up(n, 128) < 2^32; one more
-    plane is refused).  From rest, two steps disturb only the three planes
-    under the lid (the state moves one plane per step), so those must equal,
-    bitwise, the top planes of an 8-plane box of the same x-z extent, and
-    every other plane must equal that box's untouched bottom plane; the FP64
-    mass sums must differ by exactly the added rest planes."""
-    nx, ny, nz = 512, 862, 512
-    assert 19 * -(-nx * ny * nz // 128) * 128 < 2 ** 32 <= 19 * -(-nx * (ny + 1) * nz // 128) * 128
+def test_lattice_beyond_32_bit_element_indices_matches_a_short_box(storage):
+    """512 x 1600 x 512 (419 M nodes): each population buffer holds 19 x 419 M
+    elements, more than 2^32, so every kernel addresses it through windows
+    (a 64-bit base per workgroup, 32-bit offsets), and the FP32 lattice runs
+    one step per launch (the two-step kernel indexes whole buffers).  From
+    rest, two steps disturb only the three planes under the lid (the state
+    moves one plane per step), at the far end of the buffers, so those must
+    equal, bitwise, the top planes of an 8-plane box of the same x-z extent,
+    and every other plane must equal that box's untouched bottom plane; the
+    FP64 mass sums must differ by exactly the added rest planes."""
+    nx, ny, nz = 512, 1600, 512
+    assert 19 * nx * ny * nz > 2 ** 32
     # rounding to nearest: stochastic rounding seeds each node by its id, which
     # differs between the two boxes at the same place under the lid
     c = dict(nx=nx, nz=nz, enable_les=1, smagorinsky=0.17, tau=0.6, stochastic_rounding=0)
@@ -69,11 +71,13 @@ def test_largest_accepted_lattice_matches_a_short_box(storage):
     sm = small.read_macro()
     node_mass = float(np.sum(small.read_arF()[1, 1, 1].astype(np.float64)))  # a rest node, as the FP64 sum sees it
     small.close()
-    with nflow.Solver(gpu_input(cfg(ny=ny, **c), storage)) as big:
-        big.initialize()
-        big.run(2)
-        mb = big.diagnostics()[0]
-        bm = big.read_macro()
+    big = nflow.Batch([gpu_input(cfg(ny=ny, **c), storage)])
+    assert big.step2_height() == 0
+    big.initialize()
+    big.run(2)
+    mb = big.diagnostics(0)[0]
+    bm = big.read_macro(0)
+    big.close()
     for a, b in zip(bm, sm):  # [ny][nz][nx]
         assert np.array_equal(a[-3:], b[-3:])
         assert (a[:-3] == b[0]).all()
