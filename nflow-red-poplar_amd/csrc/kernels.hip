@@ -834,12 +834,17 @@ __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H
   const int tx = bt % ntx, tz = (bt / ntx) % ntz, ch = bt / (ntx * ntz);
   const int x0 = tx * TX, z0 = tz * TZ, ya = ch * H, yb = min(L.ny, ya + H);
   const int p = parity ^ L.parity0;
-  // Loads without the non-temporal hint: the halo rows, and the planes on
-  // either side of a y chunk, are loaded again by the neighbouring tile or
-  // chunk, and then find them in L2 (measured per launch, 256^3 cavity: 604
-  // -> 564 us; 512 x 128 x 128 channel: 372 -> 356 us).  The stores keep it.
-  const GPops<T, false> in{static_cast<T*>(L.buf[p])};
-  const GPops<T> out{static_cast<T*>(L.buf[p ^ 1])};
+  // The buffers are viewed from the block of the lowest plane the workgroup
+  // reads (y = ya - 2, pulled by the first step on plane ya - 1): 32-bit
+  // element indices whatever the lattice size.  Loads without the
+  // non-temporal hint: the halo rows, and the planes on either side of a y
+  // chunk, are loaded again by the neighbouring tile or chunk, and then find
+  // them in L2 (measured per launch, 256^3 cavity: 604 -> 564 us; 512 x 128
+  // x 128 channel: 372 -> 356 us).  The stores keep it.
+  constexpr uint32_t B = 256 / sizeof(T);
+  const uint32_t bn = ((uint32_t)max(ya - 2, 0) * (uint32_t)L.nxz) & ~(B - 1);
+  const GPops<T, false> in{static_cast<T*>(L.buf[p]) + window<T>(bn)};
+  const GPops<T> out{static_cast<T*>(L.buf[p ^ 1]) + window<T>(bn)};
   const int t = threadIdx.x;
   // own nodes: (x0 + lx, z0 + zrow[j]), j < K; the row group rg is the same
   // for a whole wave
@@ -856,8 +861,9 @@ __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H
   const int nlo = z0 > 0 ? rowlen : 0, nhi = z0 + TZ < L.nz ? rowlen : 0;
   const int nxl = x0 > 0 ? tzn : 0, nxr = x0 + TX < L.nx ? tzn : 0;
   const int nhalo = nlo + nhi + nxl + nxr;
+  // element index base of node (x, y, z) in the window
   auto node = [&](int x, int y, int z) {
-    return (uint32_t)x + (uint32_t)L.nx * ((uint32_t)z + (uint32_t)L.nz * (uint32_t)y);
+    return (uint32_t)x + (uint32_t)L.nx * ((uint32_t)z + (uint32_t)L.nz * (uint32_t)y) - bn;
   };
   // first step on node (x, yi, z): e = the stored values, g = them as the
   // next step reads them back
@@ -1427,8 +1433,9 @@ hipError_t launch_step(int storage, bool masked, bool sr, int block, int grid, h
 
 hipError_t launch_step2(int storage, hipStream_t st, const LatticeDev& L, int parity, int H) {
   using G = step2::Geo<STEP2_TX, STEP2_TZ, STEP2_NT>;
-  // k_step2 indexes the whole buffers with 32-bit element indices
-  if (H < 1 || storage != ST_F32 || (unsigned long long)L.pstride * Q >= (1ull << 32)) return hipErrorInvalidValue;
+  // a workgroup's window (planes ya - 2 .. ya + H + 1) takes 32-bit element indices
+  if (H < 1 || storage != ST_F32 || ((unsigned long long)(H + 4) * L.nxz + 64) * Q >= (1ull << 32))
+    return hipErrorInvalidValue;
   const int ntx = (L.nx + G::TX - 1) / G::TX, ntz = (L.nz + G::TZ - 1) / G::TZ, nch = (L.ny + H - 1) / H;
   if (L.casetype == CASE_CHANNEL)
     hipLaunchKernelGGL((k_step2<StF32, G, true>), dim3((unsigned)(ntx * ntz * nch)), dim3(G::NT), 0, st, L, parity, H,

@@ -585,9 +585,10 @@ int Batch::step2_height(const nflow_tuning* t) const {
   if (tu.temporal == 0 || storage != NFLOW_STORAGE_F32 || lats.size() != 1) return 0;
   const HostLattice& h = lats[0];
   if (h.in.casetype == NFLOW_PIPE || h.dev.mask) return 0;
-  // k_step2 indexes the whole buffers with 32-bit element indices
-  if ((unsigned long long)h.dev.pstride * 19ull >= (1ull << 32)) return 0;
-  if (tu.temporal > 0) return tu.temporal;
+  // a k_step2 workgroup reads planes ya - 2 .. ya + H + 1 through a window of
+  // 32-bit element indices
+  auto fits = [&](int hh) { return ((unsigned long long)(hh + 4) * (unsigned long long)h.dev.nxz + 64) * 19ull < (1ull << 32); };
+  if (tu.temporal > 0) return fits(tu.temporal) ? tu.temporal : 0;
   // automatic: whole tile rows in x (no idle lanes), and the fewest y chunks
   // of 16 to 64 planes whose tiles fill whole rounds of one workgroup per CU
   // (cavities of 256^3, 256 x 128 x 256 and 512 x 256 x 512 measured at
@@ -599,7 +600,7 @@ int Batch::step2_height(const nflow_tuning* t) const {
     const int hh = (h.in.ny + nch - 1) / nch;
     if (hh > 64) continue;
     if (hh < 16) break;
-    if ((cols * nch) % cu_count == 0) return hh;
+    if ((cols * nch) % cu_count == 0) return fits(hh) ? hh : 0;
   }
   return 0;
 }

@@ -54,32 +54,34 @@ def test_cavity_256_is_deterministic(storage):
 def test_lattice_beyond_32_bit_element_indices_matches_a_short_box(storage):
     """512 x 1600 x 512 (419 M nodes): each population buffer holds 19 x 419 M
     elements, more than 2^32, so every kernel addresses it through windows
-    (a 64-bit base per workgroup, 32-bit offsets), and the FP32 lattice runs
-    one step per launch (the two-step kernel indexes whole buffers).  From
-    rest, two steps disturb only the three planes under the lid (the state
-    moves one plane per step), at the far end of the buffers, so those must
-    equal, bitwise, the top planes of an 8-plane box of the same x-z extent,
-    and every other plane must equal that box's untouched bottom plane; the
-    FP64 mass sums must differ by exactly the added rest planes."""
+    (a 64-bit base per workgroup, 32-bit offsets).  The FP32 lattice runs a
+    two-step launch (y chunks of 62) and a single step, the FP16 one three
+    single steps.  From rest, three steps disturb only the four planes under
+    the lid (the state moves one plane per step), at the far end of the
+    buffers, so those must equal, bitwise, the top planes of an 8-plane box
+    of the same x-z extent, and every other plane must equal that box's
+    untouched bottom plane; the FP64 mass sums must differ by exactly the
+    added rest planes."""
     nx, ny, nz = 512, 1600, 512
     assert 19 * nx * ny * nz > 2 ** 32
     # rounding to nearest: stochastic rounding seeds each node by its id, which
     # differs between the two boxes at the same place under the lid
     c = dict(nx=nx, nz=nz, enable_les=1, smagorinsky=0.17, tau=0.6, stochastic_rounding=0)
-    small = run_gpu(cfg(ny=8, **c), 2, storage=storage)
+    steps = 3
+    small = run_gpu(cfg(ny=8, **c), steps, storage=storage)
     ms = small.diagnostics()[0]
     sm = small.read_macro()
     node_mass = float(np.sum(small.read_arF()[1, 1, 1].astype(np.float64)))  # a rest node, as the FP64 sum sees it
     small.close()
     big = nflow.Batch([gpu_input(cfg(ny=ny, **c), storage)])
-    assert big.step2_height() == 0
+    assert big.step2_height() == (62 if storage == nflow.F32 else 0)
     big.initialize()
-    big.run(2)
+    big.run(steps)
     mb = big.diagnostics(0)[0]
     bm = big.read_macro(0)
     big.close()
     for a, b in zip(bm, sm):  # [ny][nz][nx]
-        assert np.array_equal(a[-3:], b[-3:])
-        assert (a[:-3] == b[0]).all()
+        assert np.array_equal(a[-(steps + 1):], b[-(steps + 1):])
+        assert (a[:-(steps + 1)] == b[0]).all()
     assert abs(mb - (ms + (ny - 8) * nx * nz * node_mass)) < 1e-9 * mb
 
