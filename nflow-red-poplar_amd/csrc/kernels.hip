@@ -413,6 +413,34 @@ __device__ inline void moments(const float f[Q], float& rho, float& jx, float& j
   jz = c;
 }
 
+// rho and j from the opposite-pair sums s_i = f_a + f_opp(a) and differences
+// d_i = f_a - f_opp(a) (a = PA[i]), which the MRT collision works on as well:
+// 18 + 9 + 15 adds in place of the 45 of moments() plus the pairs.
+__device__ inline void pair_moments(const float f[Q], float s[NPAIR], float d[NPAIR], float& rho, float& jx, float& jy,
+                                    float& jz) {
+#pragma unroll
+  for (int i = 0; i < NPAIR; ++i) {
+    s[i] = f[PA[i]] + f[PB[i]];
+    d[i] = f[PA[i]] - f[PB[i]];
+  }
+  float r = f[0];
+#pragma unroll
+  for (int i = 0; i < NPAIR; ++i) r += s[i];
+  float a = 0.0f, b = 0.0f, c = 0.0f;
+  bool sa = false, sb = false, sc = false;
+#pragma unroll
+  for (int i = 0; i < NPAIR; ++i) {
+    const int q = PA[i];
+    if (EX[q] != 0) { a = sa ? (EX[q] > 0 ? a + d[i] : a - d[i]) : (EX[q] > 0 ? d[i] : -d[i]); sa = true; }
+    if (EY[q] != 0) { b = sb ? (EY[q] > 0 ? b + d[i] : b - d[i]) : (EY[q] > 0 ? d[i] : -d[i]); sb = true; }
+    if (EZ[q] != 0) { c = sc ? (EZ[q] > 0 ? c + d[i] : c - d[i]) : (EZ[q] > 0 ? d[i] : -d[i]); sc = true; }
+  }
+  rho = r;
+  jx = a;
+  jy = b;
+  jz = c;
+}
+
 // Smagorinsky: tau_eddy = -tau/2 + sqrt(tau^2 + 18 sqrt2 Cs^2 sqrt(Q)/rho)/2 with
 // Q = Pi:Pi and Pi = sum_q e e (f - feq) = sum_q e e (f - w rho) - j j / rho.
 __device__ inline float tau_eddy_les(const float f[Q], float rho, float ir, float jx, float jy, float jz, float tau,
@@ -481,18 +509,13 @@ __device__ inline void collide_srt(float f[Q], float rho, float ir, float jx, fl
 
 // MRT, lbm_codelets.cpp:449-537: f* = f - M^-1 S (M f - meq), with the even /
 // odd split of the orthogonal basis (each row touches only f_a + f_opp(a) or
-// only f_a - f_opp(a)) and M^-1 = M^T diag(1/|row|^2).  With LES the
-// eddy relaxation time comes from the same moments (les_from_moments).
+// only f_a - f_opp(a): s and d of pair_moments) and M^-1 = M^T diag(1/|row|^2).
+// With LES the eddy relaxation time comes from the same moments
+// (les_from_moments).
 __device__ inline float les_from_moments(const float m[Q], float rho, float ir, float jx, float jy, float jz,
                                          float tau, float les_c);
-__device__ inline void collide_mrt(float f[Q], float rho, float ir, float jx, float jy, float jz,
-                                   const LatticeDev& L) {
-  float s[NPAIR], d[NPAIR];
-#pragma unroll
-  for (int i = 0; i < NPAIR; ++i) {
-    s[i] = f[PA[i]] + f[PB[i]];
-    d[i] = f[PA[i]] - f[PB[i]];
-  }
+__device__ inline void collide_mrt(float f[Q], const float s[NPAIR], const float d[NPAIR], float rho, float ir,
+                                   float jx, float jy, float jz, const LatticeDev& L) {
   // non-conserved moments (rows 1, 2, 4, 6, 8, 9..18)
   float m[Q];
 #pragma unroll
@@ -601,24 +624,33 @@ __device__ inline void collide_mrt(float f[Q], float rho, float ir, float jx, fl
 template <bool SIDES = true>
 __device__ __forceinline__ void relax(const LatticeDev& L, float f[Q], float* __restrict__ side_out, int x, int y,
                                       int z, float* mac = nullptr) {
+  // macroscopic outputs: mac, and the inlet / outlet side records of channel
+  // and pipe lattices
+  auto outputs = [&](float rho, float ir, float jx, float jy, float jz) {
+    if (mac) {
+      mac[0] = rho;
+      mac[1] = jx * ir;
+      mac[2] = jy * ir;
+      mac[3] = jz * ir;
+    }
+    if (SIDES && L.casetype != CASE_CAVITY) {
+      const gptr<float> rec = (gptr<float>)side_out + (size_t)(y * L.nz + z) * SIDE_REC;
+      if (x == 1) rec[SIDE_RHO] = rho;
+      if (x == L.nx - 1) *(gptr<f32x4>)(rec + SIDE_U0) = f32x4{jx * ir, jy * ir, jz * ir, 0.0f};
+      if (x == L.nx - 2) *(gptr<f32x4>)(rec + SIDE_U1) = f32x4{jx * ir, jy * ir, jz * ir, 0.0f};
+    }
+  };
   float rho, jx, jy, jz;
-  moments(f, rho, jx, jy, jz);
-  const float ir = hw_rcp(rho);
-  if (mac) {
-    mac[0] = rho;
-    mac[1] = jx * ir;
-    mac[2] = jy * ir;
-    mac[3] = jz * ir;
-  }
-  if (SIDES && L.casetype != CASE_CAVITY) {  // channel and pipe: inlet / outlet side records
-    const gptr<float> rec = (gptr<float>)side_out + (size_t)(y * L.nz + z) * SIDE_REC;
-    if (x == 1) rec[SIDE_RHO] = rho;
-    if (x == L.nx - 1) *(gptr<f32x4>)(rec + SIDE_U0) = f32x4{jx * ir, jy * ir, jz * ir, 0.0f};
-    if (x == L.nx - 2) *(gptr<f32x4>)(rec + SIDE_U1) = f32x4{jx * ir, jy * ir, jz * ir, 0.0f};
-  }
   if (L.model == MODEL_MRT) {
-    collide_mrt(f, rho, ir, jx, jy, jz, L);
+    float s[NPAIR], d[NPAIR];
+    pair_moments(f, s, d, rho, jx, jy, jz);
+    const float ir = hw_rcp(rho);
+    outputs(rho, ir, jx, jy, jz);
+    collide_mrt(f, s, d, rho, ir, jx, jy, jz, L);
   } else {
+    moments(f, rho, jx, jy, jz);
+    const float ir = hw_rcp(rho);
+    outputs(rho, ir, jx, jy, jz);
     const float te = L.les ? tau_eddy_les(f, rho, ir, jx, jy, jz, L.tau, L.les_c) : 0.0f;
     collide_srt(f, rho, ir, jx, jy, jz, hw_rcp(L.tau + te));
   }
