@@ -20,6 +20,7 @@
 
 #include "d3q19.hpp"
 #include "lattice.hpp"
+#include "mrt.hpp"
 
 // Compile-time memory-access policy of the population streams.  Every
 // population is read once and written once per step and not touched again
@@ -236,18 +237,6 @@ __host__ __device__ inline uint32_t mix32(uint32_t x) {
 }
 constexpr uint32_t SR_WEYL = 0x9E3779B9u;
 
-// ---------------------------------------------------------------------------
-// Compile-time moment table
-// ---------------------------------------------------------------------------
-struct MTable {
-  int v[Q][Q];
-  constexpr MTable() : v() {
-    for (int k = 0; k < Q; ++k)
-      for (int q = 0; q < Q; ++q) v[k][q] = mrow(k, q);
-  }
-};
-constexpr MTable MT{};
-
 // e_q . (a, b, c) with only the non-zero terms of the integer vector e_q.
 __device__ inline float edot(int q, float a, float b, float c) {
   float r = 0.0f;
@@ -413,34 +402,6 @@ __device__ inline void moments(const float f[Q], float& rho, float& jx, float& j
   jz = c;
 }
 
-// rho and j from the opposite-pair sums s_i = f_a + f_opp(a) and differences
-// d_i = f_a - f_opp(a) (a = PA[i]), which the MRT collision works on as well:
-// 18 + 9 + 15 adds in place of the 45 of moments() plus the pairs.
-__device__ inline void pair_moments(const float f[Q], float s[NPAIR], float d[NPAIR], float& rho, float& jx, float& jy,
-                                    float& jz) {
-#pragma unroll
-  for (int i = 0; i < NPAIR; ++i) {
-    s[i] = f[PA[i]] + f[PB[i]];
-    d[i] = f[PA[i]] - f[PB[i]];
-  }
-  float r = f[0];
-#pragma unroll
-  for (int i = 0; i < NPAIR; ++i) r += s[i];
-  float a = 0.0f, b = 0.0f, c = 0.0f;
-  bool sa = false, sb = false, sc = false;
-#pragma unroll
-  for (int i = 0; i < NPAIR; ++i) {
-    const int q = PA[i];
-    if (EX[q] != 0) { a = sa ? (EX[q] > 0 ? a + d[i] : a - d[i]) : (EX[q] > 0 ? d[i] : -d[i]); sa = true; }
-    if (EY[q] != 0) { b = sb ? (EY[q] > 0 ? b + d[i] : b - d[i]) : (EY[q] > 0 ? d[i] : -d[i]); sb = true; }
-    if (EZ[q] != 0) { c = sc ? (EZ[q] > 0 ? c + d[i] : c - d[i]) : (EZ[q] > 0 ? d[i] : -d[i]); sc = true; }
-  }
-  rho = r;
-  jx = a;
-  jy = b;
-  jz = c;
-}
-
 // Smagorinsky: tau_eddy = -tau/2 + sqrt(tau^2 + 18 sqrt2 Cs^2 sqrt(Q)/rho)/2 with
 // Q = Pi:Pi and Pi = sum_q e e (f - feq) = sum_q e e (f - w rho) - j j / rho.
 __device__ inline float tau_eddy_les(const float f[Q], float rho, float ir, float jx, float jy, float jz, float tau,
@@ -507,41 +468,16 @@ __device__ inline void collide_srt(float f[Q], float rho, float ir, float jx, fl
   }
 }
 
-// MRT, lbm_codelets.cpp:449-537: f* = f - M^-1 S (M f - meq), with the even /
-// odd split of the orthogonal basis (each row touches only f_a + f_opp(a) or
-// only f_a - f_opp(a): s and d of pair_moments) and M^-1 = M^T diag(1/|row|^2).
-// With LES the eddy relaxation time comes from the same moments
-// (les_from_moments).
+// MRT, lbm_codelets.cpp:449-537: f* = f - M^-1 S (M f - meq) with
+// M^-1 = M^T diag(1/|row|^2) (the rows are orthogonal).  m: all 19 moments
+// (mrt_moments).  The back transform is factored like mrt_moments: each row
+// is even or odd under e -> -e, so f_a -= E + O and f_opp(a) -= E - O, with
+// the even parts sharing their axis and edge-group terms.  With LES the eddy
+// relaxation time comes from the same moments (les_from_moments).
 __device__ inline float les_from_moments(const float m[Q], float rho, float ir, float jx, float jy, float jz,
                                          float tau, float les_c);
-__device__ inline void collide_mrt(float f[Q], const float s[NPAIR], const float d[NPAIR], float rho, float ir,
-                                   float jx, float jy, float jz, const LatticeDev& L) {
-  // non-conserved moments (rows 1, 2, 4, 6, 8, 9..18)
-  float m[Q];
-#pragma unroll
-  for (int k = 0; k < Q; ++k) {
-    m[k] = 0.0f;
-    if (k == 0 || k == 3 || k == 5 || k == 7) continue;
-    float acc = 0.0f;
-    bool started = false;
-    if (!ODD[k] && MT.v[k][0] != 0) {
-      acc = (float)MT.v[k][0] * f[0];
-      started = true;
-    }
-#pragma unroll
-    for (int i = 0; i < NPAIR; ++i) {
-      const int c = MT.v[k][PA[i]];
-      if (c == 0) continue;
-      const float v = ODD[k] ? d[i] : s[i];
-      if (!started) {
-        acc = (c == 1) ? v : (c == -1 ? -v : (float)c * v);
-        started = true;
-      } else {
-        acc = (c == 1) ? acc + v : (c == -1 ? acc - v : acc + (float)c * v);
-      }
-    }
-    m[k] = acc;
-  }
+__device__ inline void collide_mrt(float f[Q], const float m[Q], float rho, float ir, float jx, float jy, float jz,
+                                   const LatticeDev& L) {
   const float snu = hw_rcp(L.tau + (L.les ? les_from_moments(m, rho, ir, jx, jy, jz, L.tau, L.les_c) : 0.0f));
   const float ir0 = L.inv_refden;
   const float jxx = jx * jx, jyy = jy * jy, jzz = jz * jz, jj = jxx + jyy + jzz;
@@ -581,40 +517,7 @@ __device__ inline void collide_mrt(float f[Q], const float s[NPAIR], const float
     else
       dl[k] = (rate * inv_n) * (m[k] - meq[k]);
   }
-  // back transform: f_a -= E + O, f_opp(a) -= E - O
-  {
-    float e0 = 0.0f;
-    bool started = false;
-#pragma unroll
-    for (int k = 1; k < Q; ++k) {
-      if (ODD[k] || k == 0 || MT.v[k][0] == 0) continue;
-      const float t = (float)MT.v[k][0] * dl[k];
-      e0 = started ? e0 + t : t;
-      started = true;
-    }
-    f[0] -= e0;
-  }
-#pragma unroll
-  for (int i = 0; i < NPAIR; ++i) {
-    float ev = 0.0f, od = 0.0f;
-    bool se = false, so = false;
-#pragma unroll
-    for (int k = 1; k < Q; ++k) {
-      if (k == 3 || k == 5 || k == 7) continue;
-      const int c = MT.v[k][PA[i]];
-      if (c == 0) continue;
-      const float t = (c == 1) ? dl[k] : (c == -1 ? -dl[k] : (float)c * dl[k]);
-      if (ODD[k]) {
-        od = so ? od + t : t;
-        so = true;
-      } else {
-        ev = se ? ev + t : t;
-        se = true;
-      }
-    }
-    f[PA[i]] -= ev + od;
-    f[PB[i]] -= ev - od;
-  }
+  mrt_back(f, dl);
 }
 
 // Macroscopic -> side planes -> LES -> collision, in place.
@@ -642,11 +545,15 @@ __device__ __forceinline__ void relax(const LatticeDev& L, float f[Q], float* __
   };
   float rho, jx, jy, jz;
   if (L.model == MODEL_MRT) {
-    float s[NPAIR], d[NPAIR];
-    pair_moments(f, s, d, rho, jx, jy, jz);
+    float m[Q];
+    mrt_moments(f, m);
+    rho = m[0];
+    jx = m[3];
+    jy = m[5];
+    jz = m[7];
     const float ir = hw_rcp(rho);
     outputs(rho, ir, jx, jy, jz);
-    collide_mrt(f, s, d, rho, ir, jx, jy, jz, L);
+    collide_mrt(f, m, rho, ir, jx, jy, jz, L);
   } else {
     moments(f, rho, jx, jy, jz);
     const float ir = hw_rcp(rho);
