@@ -68,7 +68,10 @@ namespace nflow {
 using namespace d3q19;
 
 // element index of population q of node n for element type T (ps = plane
-// stride, SoA only)
+// stride, SoA only).  (The same index as n + (Q - 1) B (n / B) + q B with one
+// 24-bit multiply-add in place of the quarter-rate 32-bit multiply measured
+// 5% faster in the two-step channel kernel but 5% slower in the 16-bit step
+// and neutral elsewhere, so the form stays.)
 template <class T>
 __device__ __forceinline__ uint32_t pidx(int q, uint32_t n, uint32_t ps) {
 #if NFLOW_AOSOA
@@ -112,19 +115,25 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 template <class T, bool NTL = (NFLOW_NT_LOAD != 0)>
 struct GPops {
   T* p;
+  // element i as a 32-bit byte offset from the uniform base: the access takes
+  // the scalar-base form, with no 64-bit address arithmetic per lane (every
+  // view is a window of less than 4 GB).  Measured, same box: two-step 256^3
+  // cavity 62.9 -> 63.9 GLUPS, channel 52.9 -> 53.1; one step within 0.5%.
+  __device__ gptr<const T> at(uint32_t i) const {
+    return (gptr<const T>)((gptr<const char>)p + i * (uint32_t)sizeof(T));
+  }
   __device__ T get(uint32_t i) const {
-    const gptr<const T> g = (gptr<const T>)p;
     if constexpr (NTL)
-      return __builtin_nontemporal_load(g + i);
+      return __builtin_nontemporal_load(at(i));
     else
-      return g[i];
+      return *at(i);
   }
   __device__ void put(uint32_t i, T v) const {
-    const gptr<T> g = (gptr<T>)p;
+    const gptr<T> g = (gptr<T>)((gptr<char>)p + i * (uint32_t)sizeof(T));
 #if NFLOW_NT_STORE
-    __builtin_nontemporal_store(v, g + i);
+    __builtin_nontemporal_store(v, g);
 #else
-    g[i] = v;
+    *g = v;
 #endif
   }
 };
@@ -1372,8 +1381,8 @@ hipError_t launch_step(int storage, bool masked, bool sr, int block, int grid, h
 
 hipError_t launch_step2(int storage, hipStream_t st, const LatticeDev& L, int parity, int H) {
   using G = step2::Geo<STEP2_TX, STEP2_TZ, STEP2_NT>;
-  // a workgroup's window (planes ya - 2 .. ya + H + 1) takes 32-bit element indices
-  if (H < 1 || storage != ST_F32 || ((unsigned long long)(H + 4) * L.nxz + 64) * Q >= (1ull << 32))
+  // a workgroup's window (planes ya - 2 .. ya + H + 1) takes 32-bit byte offsets
+  if (H < 1 || storage != ST_F32 || ((unsigned long long)(H + 4) * L.nxz + 64) * Q * 4 >= (1ull << 32))
     return hipErrorInvalidValue;
   const int ntx = (L.nx + G::TX - 1) / G::TX, ntz = (L.nz + G::TZ - 1) / G::TZ, nch = (L.ny + H - 1) / H;
   if (L.casetype == CASE_CHANNEL)

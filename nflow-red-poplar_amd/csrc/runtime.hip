@@ -586,8 +586,10 @@ int Batch::step2_height(const nflow_tuning* t) const {
   const HostLattice& h = lats[0];
   if (h.in.casetype == NFLOW_PIPE || h.dev.mask) return 0;
   // a k_step2 workgroup reads planes ya - 2 .. ya + H + 1 through a window of
-  // 32-bit element indices
-  auto fits = [&](int hh) { return ((unsigned long long)(hh + 4) * (unsigned long long)h.dev.nxz + 64) * 19ull < (1ull << 32); };
+  // 32-bit byte offsets
+  auto fits = [&](int hh) {
+    return ((unsigned long long)(hh + 4) * (unsigned long long)h.dev.nxz + 64) * 19ull * 4ull < (1ull << 32);
+  };
   if (tu.temporal > 0) return fits(tu.temporal) ? tu.temporal : 0;
   // automatic: whole tile rows in x (no idle lanes), and the fewest y chunks
   // of 16 to 64 planes whose tiles fill whole rounds of one workgroup per CU
@@ -600,7 +602,7 @@ int Batch::step2_height(const nflow_tuning* t) const {
     const int hh = (h.in.ny + nch - 1) / nch;
     if (hh > 64) continue;
     if (hh < 16) break;
-    if ((cols * nch) % cu_count == 0) return fits(hh) ? hh : 0;
+    if ((cols * nch) % cu_count == 0 && fits(hh)) return hh;
   }
   return 0;
 }
