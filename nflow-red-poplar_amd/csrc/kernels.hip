@@ -280,27 +280,50 @@ __device__ inline int node_type(const LatticeDev& L, int x, int y, int z) {
 // ---------------------------------------------------------------------------
 // `in` addresses the population array relative to a window starting at node
 // n - nr (nr = n for a whole-array view); n itself indexes the solid mask.
-template <class St, bool MASK, class Acc>
+// RUNS (AoSoA, nx a multiple of the block length B, no mask): a donor
+// e_y planes and e_z rows away lies a whole number of blocks away from the
+// node at x - e_x, so its index is one of three per-node bases (x - 1, x,
+// x + 1) plus a wave-uniform term, one add per population in place of a full
+// index computation.
+template <class St, bool MASK, class Acc, bool RUNS = false>
 __device__ inline void pull(const LatticeDev& L, const Acc& in, uint32_t n, uint32_t nr, int x, int y, int z,
                             float f[Q]) {
+  using T = typename St::T;
   const bool xlo = x == 0, xhi = x == L.nx - 1;
   const bool ylo = y == 0, yhi = y == L.ny - 1;
   const bool zlo = z == 0, zhi = z == L.nz - 1;
   const uint32_t ps = (uint32_t)L.pstride;
   const gptr<const uint8_t> mk = (gptr<const uint8_t>)L.mask;
+  static_assert(!RUNS || (!MASK && NFLOW_AOSOA), "run-aligned donors: AoSoA layout, no mask");
+  constexpr uint32_t B = 256 / sizeof(T);
+  uint32_t b0 = 0, bm = 0, bp = 0, ky = 0, kz = 0;
+  if constexpr (RUNS) {
+    b0 = pidx<T>(0, nr, ps);
+    bm = pidx<T>(0, nr - 1u, ps);
+    bp = pidx<T>(0, nr + 1u, ps);
+    ky = ((uint32_t)L.nxz / B) * (Q * B);  // elements per plane
+    kz = ((uint32_t)L.nx / B) * (Q * B);   // elements per row
+  }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     bool out = (EX[q] > 0 && xlo) || (EX[q] < 0 && xhi) || (EY[q] > 0 && ylo) || (EY[q] < 0 && yhi) ||
                (EZ[q] > 0 && zlo) || (EZ[q] < 0 && zhi);
-    // (a select of two full index computations: measured faster than
-    // selecting the source node and the plane separately, which needs more
-    // registers)
-    const int off = EX[q] + EZ[q] * L.nx + EY[q] * L.nxz;
-    if (MASK && q != 0) {
-      if (mk && !out) out = mk[(long long)n - off] != 0;
+    uint32_t idx;
+    if constexpr (RUNS) {
+      const uint32_t bx = EX[q] > 0 ? bm : (EX[q] < 0 ? bp : b0);
+      const uint32_t u = (uint32_t)q * B - (EY[q] > 0 ? ky : 0u) + (EY[q] < 0 ? ky : 0u) - (EZ[q] > 0 ? kz : 0u) +
+                         (EZ[q] < 0 ? kz : 0u);
+      idx = out ? b0 + (uint32_t)OPP[q] * B : bx + u;
+    } else {
+      // (a select of two full index computations: measured faster than
+      // selecting the source node and the plane separately, which needs more
+      // registers)
+      const int off = EX[q] + EZ[q] * L.nx + EY[q] * L.nxz;
+      if (MASK && q != 0) {
+        if (mk && !out) out = mk[(long long)n - off] != 0;
+      }
+      idx = out ? pidx<T>(OPP[q], nr, ps) : pidx<T>(q, (uint32_t)((int)nr - off), ps);
     }
-    const uint32_t idx =
-        out ? pidx<typename St::T>(OPP[q], nr, ps) : pidx<typename St::T>(q, (uint32_t)((int)nr - off), ps);
     f[q] = load_f<St>(in.get(idx), q);
   }
 }
@@ -371,11 +394,11 @@ __device__ inline void boundary(const LatticeDev& L, const SideIn& si, int x, in
   }
 }
 
-template <class St, bool MASK, class Acc>
+template <class St, bool MASK, class Acc, bool RUNS = false>
 __device__ inline void gather(const LatticeDev& L, const Acc& in, const float* __restrict__ side, uint32_t n,
                               uint32_t nr, int x, int y, int z, float f[Q]) {
   const SideIn si = side_inputs(L, side, x, y, z);
-  pull<St, MASK>(L, in, n, nr, x, y, z, f);
+  pull<St, MASK, Acc, RUNS>(L, in, n, nr, x, y, z, f);
   boundary(L, si, x, y, z, f);
 }
 
@@ -763,7 +786,9 @@ struct Geo {
 // step leaves them in LDS (two plane slots), where k_step would have written
 // the side records, and the second step writes the side records the next
 // launch reads.
-template <class St, class G, bool CH>
+// RUNS: nx is a multiple of the 64-node block, and the pulls index donors
+// from three per-node bases (pull<..., RUNS>).
+template <class St, class G, bool CH, bool RUNS>
 __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H, int ntx, int ntz) {
   static_assert(!St::shifted, "two steps per launch are built for FP32 storage");
   using namespace step2;
@@ -818,7 +843,7 @@ __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H
   auto step1 = [&](int x, int yi, int z, T e[Q], float g[Q]) {
     const uint32_t n = node(x, yi, z);
     if constexpr (CH) {
-      gather<St, false>(L, in, L.side[p], n, n, x, yi, z, g);
+      gather<St, false, decltype(in), RUNS>(L, in, L.side[p], n, n, x, yi, z, g);
       float mac[4];
       relax<false>(L, g, nullptr, x, yi, z, mac);
       if (z >= z0 && z < z0 + TZ) {  // rows of the tile: what its second step reads
@@ -828,7 +853,7 @@ __global__ void __launch_bounds__(G::NT) k_step2(LatticeDev L, int parity, int H
         if (x == L.nx - 2) { r[4] = mac[1]; r[5] = mac[2]; r[6] = mac[3]; }
       }
     } else {
-      pull<St, false>(L, in, n, n, x, yi, z, g);
+      pull<St, false, decltype(in), RUNS>(L, in, n, n, x, yi, z, g);
       boundary(L, SideIn{{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}}, x, yi, z, g);
       relax<false>(L, g, nullptr, x, yi, z);
     }
@@ -1385,12 +1410,19 @@ hipError_t launch_step2(int storage, hipStream_t st, const LatticeDev& L, int pa
   if (H < 1 || storage != ST_F32 || ((unsigned long long)(H + 4) * L.nxz + 64) * Q * 4 >= (1ull << 32))
     return hipErrorInvalidValue;
   const int ntx = (L.nx + G::TX - 1) / G::TX, ntz = (L.nz + G::TZ - 1) / G::TZ, nch = (L.ny + H - 1) / H;
-  if (L.casetype == CASE_CHANNEL)
-    hipLaunchKernelGGL((k_step2<StF32, G, true>), dim3((unsigned)(ntx * ntz * nch)), dim3(G::NT), 0, st, L, parity, H,
-                       ntx, ntz);
-  else
-    hipLaunchKernelGGL((k_step2<StF32, G, false>), dim3((unsigned)(ntx * ntz * nch)), dim3(G::NT), 0, st, L, parity,
-                       H, ntx, ntz);
+  const dim3 grid((unsigned)(ntx * ntz * nch)), block(G::NT);
+  const bool runs = L.nx % 64 == 0;  // whole 64-node blocks per row
+  if (L.casetype == CASE_CHANNEL) {
+    if (runs)
+      hipLaunchKernelGGL((k_step2<StF32, G, true, true>), grid, block, 0, st, L, parity, H, ntx, ntz);
+    else
+      hipLaunchKernelGGL((k_step2<StF32, G, true, false>), grid, block, 0, st, L, parity, H, ntx, ntz);
+  } else {
+    if (runs)
+      hipLaunchKernelGGL((k_step2<StF32, G, false, true>), grid, block, 0, st, L, parity, H, ntx, ntz);
+    else
+      hipLaunchKernelGGL((k_step2<StF32, G, false, false>), grid, block, 0, st, L, parity, H, ntx, ntz);
+  }
   return hipGetLastError();
 }
 
