@@ -68,26 +68,18 @@ def make_input(nflow, w):
 
 
 def timed_segments(k):
-    """Split k timed steps into runs of odd length: with two steps per launch
-    a run ends with one single step, and an even-length run needs two.  Four
-    (k even) or five (k odd) runs from 100 steps on, for a median; below
-    that as few runs as the parity allows (one or two), so that single steps
-    stay a small share."""
-    if k <= 5:
-        return [1] * k
-    if k < 100:
-        return [k] if k % 2 else [k // 2 - (k // 2 + 1) % 2, k // 2 + (k // 2 + 1) % 2]
-    nseg = 4 if k % 2 == 0 else 5
-    segs = [k // nseg + (1 if i < k % nseg else 0) for i in range(nseg)]
-    even = [i for i, v in enumerate(segs) if v % 2 == 0]  # an even count, since nseg = k (mod 2)
-    for a, b in zip(even[0::2], even[1::2]):
-        if segs[b] > 1:
-            segs[a] += 1
-            segs[b] -= 1
-        else:
-            segs[a] -= 1
-            segs[b] += 1
-    return [v for v in segs if v > 0]
+    """Split k timed steps into runs of even length but the last when k is
+    odd: with two steps per launch an even run is whole two-step launches
+    (with no read-back between runs the library ends it with a pair) and an
+    odd one ends with a single step.  Four near-equal runs from 100 steps on,
+    for a median; two below that (one if k < 4)."""
+    if k < 4:
+        return [k] if k else []
+    nseg = 4 if k >= 100 else 2
+    pairs, odd = divmod(k, 2)
+    segs = [2 * (pairs // nseg + (1 if i < pairs % nseg else 0)) for i in range(nseg)]
+    segs[-1] += odd
+    return segs
 
 
 def traffic_from_profiles(name, kernel="k_step"):

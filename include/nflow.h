@@ -206,8 +206,11 @@ typedef struct nflow_tuning {
   int graph_steps; /* steps per captured graph (even), 0 = no graph */
   /* Two steps per launch for a batch of one FP32 cavity or channel lattice
    * without solid nodes (other batches ignore this): x-z tiles of 256 x 8 nodes swept along
-   * y in chunks of H planes, the last launch of each run a single step, with
-   * results bit for bit those of one step per launch.  H > 0: that chunk
+   * y in chunks of H planes, with results bit for bit those of one step per
+   * launch.  A run of odd length ends with a single step; one of even length
+   * ends with a two-step launch, which a read-back of its state re-runs as two
+   * single steps (launches only), unless the previous run's state was read
+   * back: then it ends with two single steps.  H > 0: that chunk
    * height; -1 (the default): chosen by the library, on when nx is a multiple
    * of 256 and a height of 16 to 64 planes fills whole rounds of one
    * workgroup per CU; 0: one step per launch.  The autotuner times it. */

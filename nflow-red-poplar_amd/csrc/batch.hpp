@@ -29,6 +29,9 @@ struct HostLattice {
   // buf[cur ^ 1] holds the streamed state written by init/import (no step
   // launched since), rather than the previous post-collision state
   bool prev_streamed = true;
+  // the last launch was a two-step launch: buf[cur ^ 1] holds the state two
+  // steps back (the one that launch read), not the previous step's
+  bool pair_last = false;
 };
 
 int validate_input(const nflow_input& in, int storage);
@@ -67,6 +70,9 @@ class Batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int last_steps = 0;
   long long launches = 0;
+  // the state of the last run was read back: the next run of an even number
+  // of steps ends with single steps rather than a two-step launch
+  bool read_back = false;
 
   // export scratch
   float* d_aos = nullptr;
@@ -95,6 +101,11 @@ class Batch {
   int init_lattice(int idx, const float* d_src_aos, long long step0 = 0);
   int init_all();
   int launch_steps(int nsteps);  // enqueue only
+  // After a run that ended with a two-step launch, re-run that launch as two
+  // single steps from the state it read (still in buf[cur ^ 1]), so that
+  // buf[cur ^ 1] holds the previous step's state again; the same values, bit
+  // for bit.  Kernel launches only: no allocation.  No-op otherwise.
+  int settle_pair();
   int run(int nsteps);           // enqueue with timing events
   int synchronize();
   int timing(double* ms_per_step, int* steps);

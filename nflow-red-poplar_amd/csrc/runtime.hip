@@ -195,6 +195,8 @@ int Batch::open() {
 int Batch::add_lattice(const nflow_input& in, const std::vector<Tile>& tiles, void* arena_mem) {
   int rc = validate_input(in, storage);
   if (rc) return rc;
+  rc = settle_pair();
+  if (rc) return rc;
   HostLattice h;
   void* mem = arena_mem;
   const size_t bytes = lattice_bytes(in, storage);
@@ -258,6 +260,8 @@ int Batch::set_tuning(const nflow_tuning& t) {
   if (t.persistent < 0 || t.graph_steps < 0 || (t.graph_steps & 1))
     return fail(NFLOW_EINVAL, "tuning.persistent must be >= 0 and graph_steps even and >= 0");
   if (t.temporal < -1) return fail(NFLOW_EINVAL, "tuning.temporal must be >= -1");
+  int rc = settle_pair();
+  if (rc) return rc;
   tuning = t;
   tables_dirty = true;
   destroy_graphs();
@@ -346,6 +350,7 @@ int Batch::init_lattice(int idx, const float* d_src, long long step0) {
   h.cur = gparity;
   h.steps_done = step0;
   h.prev_streamed = true;
+  h.pair_last = false;
   h.initialized = true;
   return 0;
 }
@@ -386,18 +391,26 @@ int Batch::launch_steps(int nsteps) {
   int rc = upload_tables();
   if (rc) return rc;
   int left = nsteps;
-  // two steps per launch: one FP32 cavity or channel lattice without solid nodes; the
-  // last launch stays a single step so that the read-back finds the state it
-  // streams (the intermediate state of a pair never reaches memory)
+  // two steps per launch: one FP32 cavity or channel lattice without solid
+  // nodes.  The read-back streams the previous step's state, which a pair's
+  // intermediate state is not (it never reaches memory): an odd count ends
+  // with one single step, and an even count with a pair, which the read-back
+  // re-runs as two single steps (settle_pair) -- unless the previous run's
+  // state was read back, when an even count ends with two single steps
+  // (a caller that reads back after every run pays for no pair twice)
   const int h2 = step2_height();
-  if (h2 > 0 && left >= 3) {
-    const int pairs = (left - 1) / 2;
+  const bool end_single = read_back;
+  read_back = false;
+  bool pair_last = false;
+  if (h2 > 0 && left >= 2) {
+    const int pairs = (left % 2 || end_single) ? (left - 1) / 2 : left / 2;
     for (int k = 0; k < pairs; ++k) {
       NFLOW_HIP(launch_step2(storage, stream, lats[0].dev, gparity, h2));
       gparity ^= 1;
       ++launches;
     }
     left -= 2 * pairs;
+    pair_last = pairs > 0 && left == 0;
   }
   const int gs = tuning.graph_steps;
   if (gs > 0 && left >= gs && nitems > 0) {
@@ -418,9 +431,27 @@ int Batch::launch_steps(int nsteps) {
   }
   for (auto& h : lats) {
     h.steps_done += nsteps;
-    if (nsteps > 0) h.prev_streamed = false;
+    if (nsteps > 0) {
+      h.prev_streamed = false;
+      h.pair_last = pair_last;
+    }
     h.cur = gparity;
   }
+  return 0;
+}
+
+int Batch::settle_pair() {
+  if (lats.empty() || !lats[0].pair_last) return 0;
+  // a pair is launched for one lattice only, and every change of the lattice
+  // set or the tuning settles first, so the tables are those of that launch
+  if (tables_dirty || lats.size() != 1) return fail(NFLOW_ESTATE, "settle_pair: tables changed after a two-step launch");
+  const int p = gparity ^ 1;  // the parity the pair was launched with
+  NFLOW_HIP(launch_step(storage, masked, sr_any, tuning.block, grid, stream, d_lats, 1, d_items, nitems, p));
+  NFLOW_HIP(launch_step(storage, masked, sr_any, tuning.block, grid, stream, d_lats, 1, d_items, nitems, p ^ 1));
+  launches += 2;
+  gparity = p;
+  lats[0].cur = p;
+  lats[0].pair_last = false;
   return 0;
 }
 
@@ -510,6 +541,8 @@ int Batch::export_one(int idx, float* aos, float* rho, float* ux, float* uy, flo
   if (idx < 0 || idx >= (int)lats.size()) return fail(NFLOW_EINVAL, "lattice index out of range");
   HostLattice& h = lats[idx];
   if (!h.initialized) return fail(NFLOW_ESTATE, "read before initialize");
+  if (int rc = settle_pair()) return rc;
+  read_back = true;
   const size_t n = h.dev.n;
   const size_t nb = (n + export_block_size() - 1) / export_block_size();
   float* macro = nullptr;
@@ -902,6 +935,9 @@ extern "C" int nflow_diagnostics_noalloc(nflow_solver* s, double* mass, double m
   double* ptr = nullptr;
   int rc = scratch_on(b.device, &ptr);
   if (rc) return rc;
+  rc = b.settle_pair();
+  if (rc) return rc;
+  b.read_back = true;
   double* const partials = ptr;
   double* const sums = ptr + (NOALLOC_MAX_NODES / 256) * 5;
   // the kernels, grid and order of nflow_diagnostics: the same sums, bit for

@@ -2,8 +2,10 @@
 channel lattices) against one step per launch: the same state bit for bit,
 for both collision models, partial tiles and y chunks, several x tiles, the
 channel's inlet and outlet in one or two tiles, tiny boxes, the benchmark
-size, split runs, and a read-back after every run (the last launch of a run
-is a single step, so the export finds the state it streams)."""
+size, split runs, a read-back after every run, and read-backs after runs that
+end with a two-step launch (the read-back streams the previous step's state,
+which such a launch does not leave in memory: it is re-run as two single
+steps first)."""
 import numpy as np
 import pytest
 
@@ -147,3 +149,56 @@ def test_automatic_chunk_height():
     assert height(512, 128, 128, casetype=nflow.PIPE) == 0
     assert height(256, 256, 256, temporal=24) == 24
     assert height(256, 256, 256, temporal=0) == 0
+
+
+@pytest.mark.parametrize("k", [0, 7])
+def test_read_back_after_runs_that_end_with_a_pair(k, tmp_path):
+    """Runs of even length with no read-back between end with a two-step
+    launch, after which the other buffer holds the state two steps back; each
+    kind of read-back (macro fields, populations, diagnostics, the
+    allocation-free diagnostics, a checkpoint, a change of tuning) first
+    re-runs that launch as two single steps and must find the state of one
+    step per launch, bit for bit, and so must the runs after it."""
+    c = CASES[k]
+
+    def sequence(temporal):
+        s = nflow.Solver(nflow.defaults(**c, storage=nflow.F32))
+        s.set_tuning(256, 0, 0, 0, temporal)
+        s.initialize()
+        out = []
+        s.run(10)
+        s.run(6)  # ends with a pair (h > 0)
+        out.append(("macro", s.read_macro()))
+        s.run(4)  # after a read-back: ends with single steps
+        out.append(("arF", s.read_arF()))
+        s.run(2)
+        s.run(2)
+        out.append(("noalloc", s.diagnostics_noalloc()))
+        s.run(8)
+        out.append(("diag", s.diagnostics()))
+        s.run(12)
+        s.run(2)
+        path = str(tmp_path / f"t{temporal}.ckp")
+        s.save_checkpoint(path)
+        with open(path, "rb") as f:
+            out.append(("checkpoint", f.read()))
+        s.run(4)  # after a read-back: ends with single steps
+        s.run(2)  # ends with a pair
+        s.set_tuning(256, 0, 0, 0, 0)  # settles the pair before the tuning changes
+        s.run(3)
+        out.append(("after retune", s.read_arF()))
+        out.append(("steps", s.steps_done()))
+        s.close()
+        return out
+
+    ref = sequence(0)
+    for h in (1, 32):
+        got = sequence(h)
+        for (name, a), (_, b) in zip(got, ref):
+            if isinstance(a, tuple):
+                for x, y in zip(a, b):
+                    assert np.array_equal(np.asarray(x), np.asarray(y)), (h, name)
+            elif isinstance(a, np.ndarray):
+                assert np.array_equal(a, b), (h, name)
+            else:
+                assert a == b, (h, name)
