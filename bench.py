@@ -72,12 +72,12 @@ def timed_segments(k):
     odd: with two steps per launch an even run is whole two-step launches
     (with no read-back between runs the library ends it with a pair) and an
     odd one ends with a single step.  Four near-equal runs from 100 steps on,
-    for a median; two below that (one if k < 4)."""
-    if k < 4:
+    for a median; one below that, since reading a run's device time waits for
+    it and leaves the GPU idle until the next run is launched."""
+    if k < 100:
         return [k] if k else []
-    nseg = 4 if k >= 100 else 2
     pairs, odd = divmod(k, 2)
-    segs = [2 * (pairs // nseg + (1 if i < pairs % nseg else 0)) for i in range(nseg)]
+    segs = [2 * (pairs // 4 + (1 if i < pairs % 4 else 0)) for i in range(4)]
     segs[-1] += odd
     return segs
 

@@ -24,8 +24,8 @@ def test_timed_segments_are_even_and_sum_to_k():
         assert sum(v % 2 for v in segs) == k % 2, (k, segs)
         assert len(segs) <= 4, (k, segs)
     assert b.timed_segments(2000) == [500, 500, 500, 500]
-    assert b.timed_segments(20) == [10, 10]
-    assert b.timed_segments(21) == [10, 11]
+    assert b.timed_segments(20) == [20]
+    assert b.timed_segments(21) == [21]
     # near-equal runs: the K = 2000 default's median segment is a full run
     for k in (200, 2000, 2001, 20000):
         segs = b.timed_segments(k)
