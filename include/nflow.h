@@ -211,9 +211,10 @@ typedef struct nflow_tuning {
    * ends with a two-step launch, which a read-back of its state re-runs as two
    * single steps (launches only), unless the previous run's state was read
    * back: then it ends with two single steps.  H > 0: that chunk
-   * height; -1 (the default): chosen by the library, on when nx is a multiple
-   * of 256 and a height of 16 to 64 planes fills whole rounds of one
-   * workgroup per CU; 0: one step per launch.  The autotuner times it. */
+   * height; -1 (the default): chosen by the library from an estimate of the
+   * launch time at heights 8 to 64, on where the estimate beats one step per
+   * launch (cavities from about 176^3 up, channels of 128 x 128 cross-section
+   * from 256 long); 0: one step per launch.  The autotuner times it. */
   int temporal;
   double ms_per_step; /* filled by the autotuner */
 } nflow_tuning;

@@ -624,20 +624,39 @@ int Batch::step2_height(const nflow_tuning* t) const {
     return ((unsigned long long)(hh + 4) * (unsigned long long)h.dev.nxz + 64) * 19ull * 4ull < (1ull << 32);
   };
   if (tu.temporal > 0) return fits(tu.temporal) ? tu.temporal : 0;
-  // automatic: whole tile rows in x (no idle lanes), and the fewest y chunks
-  // of 16 to 64 planes whose tiles fill whole rounds of one workgroup per CU
-  // (cavities of 256^3, 256 x 128 x 256 and 512 x 256 x 512 measured at
-  // 1.31-1.43x the one-step rate, channels at 1.13-1.25x; with idle lanes or
-  // a partial last round it can be slower)
-  if (h.in.nx % STEP2_TX != 0) return 0;
-  const long cols = (long)(h.in.nx / STEP2_TX) * ((h.in.nz + STEP2_TZ - 1) / STEP2_TZ);
-  for (int nch = 1; nch <= h.in.ny; ++nch) {
-    const int hh = (h.in.ny + nch - 1) / nch;
-    if (hh > 64) continue;
-    if (hh < 16) break;
-    if ((cols * nch) % cu_count == 0 && fits(hh)) return hh;
+  // automatic: the chunk height of the least estimated launch time, and two
+  // steps per launch only where that estimate beats 40 G node updates per
+  // second (one step per launch runs at 37-39.5 from 128^3 up).  Estimate:
+  // one workgroup per CU (its LDS), each sweeping its chunk's planes plus
+  // 0.86 of halo and pipeline overhead at 12.18 + 2.70 w us per plane, w the
+  // share of the CUs busy in its round (memory contention); times
+  // 0.84 + 0.16 fill for the idle lanes of partial x tiles (fill = nx over
+  // the tiles' width) and 1.21 for a channel's boundary records.  Fitted to
+  // the measured rates of 34 cavities and channels from 48^3 to 512^3 at
+  // heights 8-64 (8% rms); it picks the fastest height in every one of them.
+  const int ntx = (h.in.nx + STEP2_TX - 1) / STEP2_TX;
+  const long cols = (long)ntx * ((h.in.nz + STEP2_TZ - 1) / STEP2_TZ);
+  const double fill = (double)h.in.nx / ((double)ntx * STEP2_TX);
+  const double scale = (0.841 + 0.159 * fill) * (h.in.casetype == NFLOW_CHANNEL ? 1.213 : 1.0);
+  const double nodes = (double)h.in.nx * h.in.ny * h.in.nz;
+  double best_us = 2.0 * nodes / 40.0e3;  // two steps at 40 nodes per ns
+  int best = 0, prev = 0;
+  for (int cand : {8, 12, 16, 24, 32, 48, 64}) {
+    const int hh = std::min(cand, h.in.ny);
+    if (hh == prev || !fits(hh)) continue;
+    prev = hh;
+    const long nch = (h.in.ny + hh - 1) / hh;
+    const double planes = (double)h.in.ny / nch + 0.858;
+    double us = 0.0;
+    for (long left = cols * nch; left > 0; left -= cu_count)
+      us += planes * (12.184 + 2.696 * (double)std::min<long>(left, cu_count) / cu_count);
+    us *= scale;
+    if (us < best_us) {
+      best_us = us;
+      best = hh;
+    }
   }
-  return 0;
+  return best;
 }
 
 // Times every candidate launch shape on this batch's own lattices and keeps
@@ -690,9 +709,9 @@ int Batch::autotune(int trial_steps, nflow_tuning* out) {
   probe.temporal = 16;
   if (step2_height(&probe) > 0) {
     probe.temporal = -1;
-    std::vector<int> hs = {16, 32, 64};
+    std::vector<int> hs = {8, 16, 24, 32, 64};
     const int ah = step2_height(&probe);
-    if (ah > 0 && ah != 16 && ah != 32 && ah != 64) hs.push_back(ah);
+    if (ah > 0 && std::find(hs.begin(), hs.end(), ah) == hs.end()) hs.push_back(ah);
     std::vector<nflow_tuning> tc;
     for (int hh : hs) {
       nflow_tuning t = best;

@@ -129,9 +129,11 @@ def test_temporal_is_ignored_where_it_does_not_apply():
 
 
 def test_automatic_chunk_height():
-    """temporal = -1 (the default): two steps per launch where nx fills whole
-    256-wide tiles and a chunk height of 16-64 planes fills whole rounds of
-    one workgroup per CU (256 CUs)."""
+    """temporal = -1 (the default): the chunk height (8-64 planes) of the least
+    estimated launch time, two steps per launch only where the estimate beats
+    40 G node updates per second (256 CUs).  Each case's height is the fastest
+    measured among 8, 12, 16, 24, 32, 48 and 64, and each case left at one
+    step per launch measured no faster with two (within 1%; 128^3 7.6%)."""
     def height(nx, ny, nz, storage=nflow.F32, casetype=nflow.CAVITY, temporal=-1):
         b = nflow.Batch([nflow.defaults(**cfg(nx=nx, ny=ny, nz=nz, casetype=casetype), storage=storage)])
         b.set_tuning(512, 0, 0, 0, temporal)
@@ -142,10 +144,21 @@ def test_automatic_chunk_height():
     assert height(256, 256, 256) == 32  # 32 tiles x 8 chunks
     assert height(256, 128, 256) == 16
     assert height(512, 256, 512) == 64  # 128 tiles x 4 chunks
-    assert height(128, 128, 128) == 0  # idle lanes
-    assert height(512, 64, 64) == 0  # no chunking fills whole rounds
+    assert height(256, 64, 256) == 8
+    assert height(256, 96, 256) == 12
+    assert height(192, 192, 192) == 24  # partial x tiles
+    assert height(384, 384, 384) == 48
+    assert height(200, 300, 100) == 16
+    assert height(128, 128, 128) == 0  # half the lanes idle
+    assert height(160, 160, 160) == 0
+    assert height(288, 288, 288) == 0  # second x tile an eighth full
+    assert height(512, 64, 64) == 0  # too few tiles for the CUs
+    assert height(48, 48, 48) == 0
     assert height(256, 256, 256, storage=nflow.F16) == 0
     assert height(512, 128, 128, casetype=nflow.CHANNEL) == 16  # 32 tiles x 8 chunks
+    assert height(768, 128, 128, casetype=nflow.CHANNEL) == 8
+    assert height(256, 128, 128, casetype=nflow.CHANNEL) == 8
+    assert height(320, 128, 128, casetype=nflow.CHANNEL) == 0
     assert height(512, 128, 128, casetype=nflow.PIPE) == 0
     assert height(256, 256, 256, temporal=24) == 24
     assert height(256, 256, 256, temporal=0) == 0
