@@ -739,7 +739,9 @@ __global__ void __launch_bounds__(256) k_advance(const LatticeDev* __restrict__ 
 // bit; the memory traffic of the pair is one read and one write of the
 // populations (plus the halo) instead of two of each.  The intermediate state
 // never reaches memory, so the launch before a read-back must be a k_step
-// (the export streams the state the last launch read).
+// (the export streams the state the last launch read): a run that ends with
+// a k_step2 is settled by re-running it as two k_step launches from the state
+// it read, which it leaves untouched.
 // ---------------------------------------------------------------------------
 namespace step2 {
 // Slot of crossing population q (e_x or e_z non-zero) in an LDS row: the 5
