@@ -682,9 +682,11 @@ int Batch::autotune(int trial_steps, nflow_tuning* out) {
         t.ms_per_step = 1e30;
         cand.push_back(t);
       }
+  // initialize before set_tuning: a fresh state has no two-step launch for
+  // set_tuning to settle
   auto trial = [&](nflow_tuning& t, int steps) -> int {
-    int rc = set_tuning(t);
-    if (!rc) rc = init_all();
+    int rc = init_all();
+    if (!rc) rc = set_tuning(t);
     if (!rc) rc = launch_steps(2);
     if (!rc) rc = run(steps);
     double ms = 0.0;
@@ -728,11 +730,11 @@ int Batch::autotune(int trial_steps, nflow_tuning* out) {
       if (t.ms_per_step < best.ms_per_step) best = t;
   }
   best.graph_steps = base.graph_steps;
+  rc = init_all();
+  if (rc) return rc;
   rc = set_tuning(best);
   if (rc) return rc;
   tuning.ms_per_step = best.ms_per_step;
-  rc = init_all();
-  if (rc) return rc;
   NFLOW_HIP(hipStreamSynchronize(stream));
   if (out) *out = tuning;
   return 0;
