@@ -53,7 +53,7 @@ def test_two_steps_per_launch_equal_one(k):
     c = CASES[k]
     steps = [7, 8, 3, 20]
     ref = run(c, nflow.F32, 0, steps, 0)
-    for h in (1, 3, 32):
+    for h in (1, 3, 8, 12, 24, 32, 64):  # 8-64: the heights the automatic rule picks
         got = run(c, nflow.F32, 0, steps, h)
         for j, ((a, m, d), (a0, m0, d0)) in enumerate(zip(got, ref)):
             assert np.array_equal(a, a0), (h, j)
